@@ -1,0 +1,298 @@
+"""ctypes wrapper of libpmoracle.so (the C restatement) and of the
+reference-subset library _ref/libpmref.so (the reference's own Fortran
+element, promotion, numbering and assembly routines, built from the
+reference sources when they are present).
+
+TEST INFRASTRUCTURE ONLY -- see the package docstring.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+NEN, NENV, NDF, NST, PROP = 20, 8, 4, 68, 15
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class OracleEnv(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double),
+        ("t", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("gravity_mod", C.c_double),
+        ("load", C.c_void_p),
+        ("fload", C.c_void_p),
+    ]
+
+
+class _Model(C.Structure):
+    _fields_ = [
+        ("numel", C.c_int), ("nnode", C.c_int), ("nnodev", C.c_int), ("neq", C.c_int),
+        ("nad", C.c_int64),
+        ("ix", C.c_void_p), ("x", C.c_void_p), ("id", C.c_void_p), ("fnno", C.c_void_p),
+        ("e", C.c_void_p), ("eload", C.c_void_p), ("eloadp", C.c_void_p),
+        ("nload", C.c_void_p), ("f", C.c_void_p),
+        ("ia", C.c_void_p), ("ja", C.c_void_p), ("ad", C.c_void_p), ("al", C.c_void_p),
+        ("u", C.c_void_p), ("u0", C.c_void_p), ("dp", C.c_void_p), ("porosity", C.c_void_p),
+        ("b0", C.c_void_p), ("b", C.c_void_p), ("x0", C.c_void_p), ("m", C.c_void_p),
+    ]
+
+
+class _Cfg(C.Structure):
+    _fields_ = [
+        ("solver", C.c_int), ("precond", C.c_int), ("maxit", C.c_int), ("ngram", C.c_int),
+        ("maxnlit", C.c_int), ("stop_crit", C.c_int), ("solvtol", C.c_double),
+        ("tol", C.c_double),
+    ]
+
+
+class _StepInfo(C.Structure):
+    _fields_ = [
+        ("nlit", C.c_int), ("nsolve", C.c_int), ("iters", C.c_int * 16),
+        ("used", C.c_int * 16), ("resid", C.c_double * 16),
+    ]
+
+
+def _load_lib():
+    path = HERE / "libpmoracle.so"
+    if not path.exists():
+        raise RuntimeError(f"oracle library missing: build it with `make -C {HERE}`")
+    L = C.CDLL(str(path))
+    L.orc_elmt17.restype = C.c_int
+    L.orc_elmt17.argtypes = [_f64p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_int,
+                             C.POINTER(OracleEnv)]
+    L.orc_promote_hexa20.restype = C.c_int
+    L.orc_promote_hexa20.argtypes = [_i32p, C.c_int, C.c_int]
+    L.orc_bound_quad.argtypes = [_i32p, _i32p, C.c_int]
+    L.orc_graph_count.restype = C.c_int64
+    L.orc_graph_count.argtypes = [_i32p, C.c_int, C.c_int]
+    L.orc_graph.argtypes = [_i32p, C.c_int, C.c_int, _i64p, _i32p, _i32p]
+    L.orc_genrcm.argtypes = [C.c_int, _i64p, _i32p, _i32p]
+    L.orc_numeq.restype = C.c_int
+    L.orc_numeq.argtypes = [_i32p, _i32p, _i32p, C.c_int]
+    L.orc_csrc_count.restype = C.c_int64
+    L.orc_csrc_count.argtypes = [_i32p, _i32p, _i64p, _i32p, C.c_int, C.c_int, _i64p]
+    L.orc_csrc_fill.argtypes = [_i32p, _i32p, _i64p, _i32p, C.c_int, C.c_int, _i64p, _i32p]
+    L.orc_matvec_csrc.argtypes = [C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_dot.restype = C.c_double
+    L.orc_dot.argtypes = [_f64p, _f64p, C.c_int]
+    L.orc_precond.argtypes = [C.c_int, C.c_int, _f64p, _f64p]
+    sig = [C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_pcg.restype = C.c_int
+    L.orc_pcg.argtypes = sig + [C.c_double, C.c_int, C.POINTER(C.c_double)]
+    L.orc_rpsqmr.restype = C.c_int
+    L.orc_rpsqmr.argtypes = sig + [C.c_double, C.c_int, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int)]
+    L.orc_pbicgstab.restype = C.c_int
+    L.orc_pbicgstab.argtypes = sig + [C.c_double, C.c_int, C.POINTER(C.c_double)]
+    L.orc_pminres.restype = C.c_int
+    L.orc_pminres.argtypes = sig + [C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.orc_gmres2.restype = C.c_int
+    L.orc_gmres2.argtypes = sig + [C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double)]
+    L.orc_pform.restype = C.c_int
+    L.orc_pform.argtypes = [C.POINTER(_Model), C.c_int, C.c_int, C.c_int, _f64p,
+                            C.POINTER(OracleEnv)]
+    L.orc_step.restype = C.c_int
+    L.orc_step.argtypes = [C.POINTER(_Model), C.POINTER(_Cfg), C.POINTER(OracleEnv),
+                           C.POINTER(_StepInfo)]
+    L.orc_tload.restype = C.c_double
+    L.orc_tload.argtypes = [C.c_int, C.c_double, _f64p, C.c_double, _f64p,
+                            C.POINTER(OracleEnv)]
+    return L
+
+
+lib = _load_lib()
+
+
+def make_env(load, fload, dt, t=0.0, gravity=(0.0, 0.0, 0.0), gravity_mod=10.0):
+    load = np.ascontiguousarray(load, dtype=np.int32)
+    fload = np.ascontiguousarray(fload, dtype=np.float64)
+    env = OracleEnv()
+    env.dt = dt
+    env.t = t
+    env.gravity[:] = list(gravity)
+    env.gravity_mod = gravity_mod
+    env.load = load.ctypes.data
+    env.fload = fload.ctypes.data
+    env._keep = (load, fload)
+    return env
+
+
+def elmt17(e, iqm, iqh, xl, ul, dpl, isw, env, p=None):
+    """One elmt17_pm call; returns (rc, p[68], s[68,68] as s[i, j])."""
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    iqm = np.ascontiguousarray(iqm, dtype=np.int32)
+    iqh = np.ascontiguousarray(iqh, dtype=np.int32)
+    xl = np.ascontiguousarray(xl, dtype=np.float64).reshape(-1)
+    ul = np.ascontiguousarray(ul, dtype=np.float64)
+    dpl = np.ascontiguousarray(dpl, dtype=np.float64)
+    p = np.zeros(NST) if p is None else np.ascontiguousarray(p, dtype=np.float64).copy()
+    s = np.zeros(NST * NST)
+    rc = lib.orc_elmt17(e, iqm, iqh, xl, ul, dpl, p, s, isw, C.byref(env))
+    return rc, p, s.reshape(NST, NST).T  # s stored column-major
+
+
+def build_structure(deck, reorder=True):
+    """Runs promotion, mk_bound_quad, graph, RCM, numbering and the CSRC
+    pattern exactly as the mesh macro does.  Returns a dict of arrays."""
+    numel, nnodev = deck.numel, deck.nnodev
+    ix = np.ascontiguousarray(deck.ix, dtype=np.int32).copy()
+    nnode = lib.orc_promote_hexa20(ix, numel, nnodev)
+    idf = np.zeros((nnode, NDF), dtype=np.int32)
+    idf[:nnodev] = deck.idv
+    lib.orc_bound_quad(idf, ix, numel)
+    fnno = np.zeros(nnode, dtype=np.int32)
+    fnno[:nnodev] = 1
+    ipos = lib.orc_graph_count(ix, numel, nnode)
+    ip = np.zeros(nnode + 1, dtype=np.int64)
+    ips = np.zeros(max(ipos, 1), dtype=np.int32)
+    work = np.zeros(max(ipos, 1), dtype=np.int32)
+    lib.orc_graph(ix, numel, nnode, ip, ips, work)
+    ips = ips[: ip[-1] - 1].copy()
+    if reorder:
+        perm = np.zeros(nnode, dtype=np.int32)
+        lib.orc_genrcm(nnode, ip, ips, perm)
+    else:
+        perm = np.arange(1, nnode + 1, dtype=np.int32)
+    idflags = idf.copy()
+    neq = lib.orc_numeq(idf, perm, fnno, nnode)
+    ia = np.zeros(neq + 1, dtype=np.int64)
+    nad = lib.orc_csrc_count(idf, perm, ip, ips, nnode, neq, ia)
+    ja = np.zeros(max(nad, 1), dtype=np.int32)
+    lib.orc_csrc_fill(idf, perm, ip, ips, nnode, neq, ia, ja)
+    return dict(ix=ix, nnode=nnode, nnodev=nnodev, numel=numel, idflags=idflags, id=idf,
+                fnno=fnno, ip=ip, ips=ips, perm=perm, neq=neq, nad=int(nad), ia=ia,
+                ja=ja[:nad])
+
+
+class OracleModel:
+    """All state of the oracle's time march for one deck."""
+
+    def __init__(self, deck, st=None, solver=11, precond=2, solvtol=1e-10, maxit=50000,
+                 ngram=150, maxnlit=500, tol=1e-4, stop_crit=1):
+        self.deck = deck
+        self.st = st if st is not None else build_structure(deck)
+        st = self.st
+        nnode, neq, nad = st["nnode"], st["neq"], st["nad"]
+        self.e = np.ascontiguousarray(deck.e, dtype=np.float64)
+        self.eload = np.ascontiguousarray(deck.eload, dtype=np.int32)
+        self.eloadp = np.ascontiguousarray(deck.eloadp, dtype=np.int32)
+        self.x = np.zeros((nnode, 3))
+        self.x[: deck.nnodev] = deck.x
+        self.nload = np.zeros((nnode, NDF), dtype=np.int32)
+        self.f = np.zeros((nnode, NDF))
+        self.ad = np.zeros(neq)
+        self.al = np.zeros(max(nad, 1))
+        self.u = np.zeros((nnode, NDF))
+        self.u0 = np.zeros((nnode, NDF))
+        self.dp = np.zeros(nnode)
+        self.porosity = np.zeros(nnode)
+        self.b0 = np.zeros(neq)
+        self.b = np.zeros(neq)
+        self.x0 = np.zeros(neq)
+        self.m = np.zeros(neq)
+        self.env = make_env(deck.load, deck.fload, deck.dt, 0.0, deck.gravity, deck.gravity_mod)
+        self.cfg = _Cfg(solver, precond, maxit, ngram, maxnlit, stop_crit, solvtol, tol)
+        self._md = _Model()
+        md = self._md
+        md.numel, md.nnode, md.nnodev, md.neq, md.nad = (st["numel"], nnode, deck.nnodev, neq,
+                                                         nad)
+        for name, arr in (("ix", st["ix"]), ("x", self.x), ("id", st["id"]), ("fnno", st["fnno"]),
+                          ("e", self.e), ("eload", self.eload), ("eloadp", self.eloadp),
+                          ("nload", self.nload), ("f", self.f), ("ia", st["ia"]),
+                          ("ja", st["ja"]), ("ad", self.ad), ("al", self.al), ("u", self.u),
+                          ("u0", self.u0), ("dp", self.dp), ("porosity", self.porosity),
+                          ("b0", self.b0), ("b", self.b), ("x0", self.x0), ("m", self.m)):
+            setattr(md, name, arr.ctypes.data)
+        self.t = 0.0
+
+    def assemble(self, isw=2, lhs=True, rhs=True, bvec=None):
+        bvec = self.b if bvec is None else bvec
+        rc = lib.orc_pform(C.byref(self._md), isw, int(lhs), int(rhs), bvec, C.byref(self.env))
+        if rc:
+            raise RuntimeError(f"orc_pform failed: {rc}")
+        return bvec
+
+    def step(self):
+        info = _StepInfo()
+        rc = lib.orc_step(C.byref(self._md), C.byref(self.cfg), C.byref(self.env), C.byref(info))
+        if rc:
+            raise RuntimeError(f"orc_step failed: {rc}")
+        self.t = self.env.t
+        n = min(info.nsolve, 16)
+        return dict(nlit=info.nlit, nsolve=info.nsolve, iters=list(info.iters[:n]),
+                    used=list(info.used[:n]), resid=list(info.resid[:n]))
+
+
+# ---------------------------------------------------------------------------
+# reference-subset library (built from the reference's own sources)
+# ---------------------------------------------------------------------------
+_ref = None
+
+
+def load_ref():
+    """The reference's element / promotion / numbering / assembly routines
+    compiled from /root/reference by ``make -C oracle ref``; None if absent."""
+    global _ref
+    if _ref is not None:
+        return _ref
+    path = HERE / "_ref" / "libpmref.so"
+    if not path.exists():
+        return None
+    # lazy binding: the library leaves symbols of files that need MPI
+    # unresolved (stop_mef, vet, ...); none is on the paths called here
+    _ref = C.CDLL(str(path), mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    return _ref
+
+
+def _common(R, name, ctype):
+    return ctype.in_dll(R, name)
+
+
+def ref_set_common(R, dt, t, gravity, gravity_mod, load, fload):
+    transi = (C.c_double * 5).in_dll(R, "transi_")   # alfa, beta, dt, t, dt_mec
+    transi[2] = dt
+    transi[3] = t
+    mec = (C.c_double * 4).in_dll(R, "mecprop_")     # gravity(3), gravity_mod
+    mec[0], mec[1], mec[2] = gravity
+    mec[3] = gravity_mod
+    nload = 3 * 200
+    ld = (C.c_int * (nload + 1000 * 5 * 200 * 2)).in_dll(R, "load_")
+    buf = np.frombuffer(ld, dtype=np.uint8)
+    buf[: nload * 4] = np.ascontiguousarray(load, dtype=np.int32).view(np.uint8).ravel()
+    buf[nload * 4: nload * 4 + 1000 * 5 * 200 * 8] = np.ascontiguousarray(
+        fload, dtype=np.float64).view(np.uint8).ravel()
+
+
+def ref_elmt17(e, iqm, iqh, xl, ul, dpl, isw, dt, t=0.0, gravity=(0.0, 0.0, 0.0),
+               gravity_mod=10.0, load=None, fload=None, p=None):
+    """Calls the reference's elmt17_pm directly.  Returns (p, s[i, j])."""
+    R = load_ref()
+    if R is None:
+        raise RuntimeError("reference-subset library not built")
+    if load is None:
+        load = np.zeros((200, 3), dtype=np.int32)
+        fload = np.zeros((200, 5, 1000))
+    ref_set_common(R, dt, t, gravity, gravity_mod, load, fload)
+    e = np.ascontiguousarray(e, dtype=np.float64).copy()
+    iqm = np.ascontiguousarray(iqm, dtype=np.int32).copy()
+    iqh = np.ascontiguousarray(iqh, dtype=np.int32).copy()
+    xl = np.ascontiguousarray(xl, dtype=np.float64).reshape(-1).copy()
+    ul = np.ascontiguousarray(ul, dtype=np.float64).copy()
+    dpl = np.ascontiguousarray(dpl, dtype=np.float64).copy()
+    p = np.zeros(NST) if p is None else np.ascontiguousarray(p, dtype=np.float64).copy()
+    s = np.zeros(NST * NST)
+    txn = np.zeros(6 * NEN)
+    ndm, nst, nel, isw_, block_pu = C.c_int(3), C.c_int(NST), C.c_int(1), C.c_int(isw), C.c_int(0)
+    dptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    R.elmt17_pm_(dptr(e), dptr(iqm), dptr(iqh), dptr(xl), dptr(ul), dptr(dpl), dptr(p), dptr(s),
+                 dptr(txn), C.byref(ndm), C.byref(nst), C.byref(nel), C.byref(isw_),
+                 C.byref(block_pu))
+    return p, s.reshape(NST, NST).T

@@ -1,0 +1,133 @@
+/*
+ * pm_oracle.h -- CPU restatement of poroMec's `mef` Biot u-p solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness checker for the
+ * HIP implementation in poromec_amd/.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it, and only as the checker or the
+ * timed CPU baseline -- never as part of the product path.
+ *
+ * Conventions follow the reference (Fortran, column-major, 1-based node and
+ * equation numbers):
+ *   ix[(NEN+1)*e + k]   element e (0-based), local node k (0..19), value is a
+ *                       1-based node id; ix[(NEN+1)*e + 20] is the 1-based
+ *                       material id                          (Rdata.f elconn)
+ *   x[3*n + d]          vertex coordinates, n 0-based        (Rdata.f coord)
+ *   id[NDF*n + j]       restraint flag (1 = fixed) before numbering and the
+ *                       1-based equation number (0 = none) after it
+ *   u[NDF*n + j]        nodal unknowns, j = 0..2 displacement, 3 pressure
+ *   s[NST*j + i]        element matrix s(i,j)
+ *   ia[0..neq]          1-based int64 row pointers of the strict lower CSRC
+ *   ja[0..nad-1]        1-based int32 columns
+ */
+#ifndef PM_ORACLE_H
+#define PM_ORACLE_H
+#include <stdint.h>
+
+#define ORC_NEN 20
+#define ORC_NENV 8
+#define ORC_NDF 4
+#define ORC_NST 68
+#define ORC_PROP 15
+#define ORC_NUMLOAD 200
+#define ORC_NTERMS 5
+#define ORC_NPAR 1000
+
+typedef struct {
+  double dt, t;
+  double gravity[3];
+  double gravity_mod;
+  const int *load;     /* load(3, ORC_NUMLOAD), column-major, load.fi */
+  const double *fload; /* fload(ORC_NPAR, ORC_NTERMS, ORC_NUMLOAD), load.fi */
+} orc_env;
+
+typedef struct {
+  int numel, nnode, nnodev, neq;
+  int64_t nad;
+  const int *ix;      /* (NEN+1) x numel */
+  const double *x;    /* 3 x nnodev */
+  const int *id;      /* NDF x nnode, equation numbers */
+  const int *fnno;    /* nnode: 1 for vertices */
+  const double *e;    /* PROP x numat */
+  const int *eload;   /* 7 x numel, mechanical face-load ids */
+  const int *eloadp;  /* 7 x numel, hydraulic face-load ids */
+  const int *nload;   /* NDF x nnode nodal load ids */
+  double *f;          /* NDF x nnode nodal load / prescribed values */
+  const int64_t *ia;  /* neq+1 */
+  const int *ja;      /* nad */
+  double *ad, *al;    /* neq, nad */
+  double *u, *u0;     /* NDF x nnode */
+  double *dp;         /* nnode */
+  double *porosity;   /* nnode */
+  double *b0, *b, *x0, *m; /* neq */
+} orc_model;
+
+typedef struct {
+  int solver;   /* 1 pcg, 2 gmres, 4 bicgstab, 7 minres, 11 rpsqmr */
+  int precond;  /* 2 diag, 5 |diag| */
+  int maxit, ngram, maxnlit, stop_crit;
+  double solvtol, tol;
+} orc_cfg;
+
+typedef struct {
+  int nlit;          /* nonlinear iterations taken this step */
+  int nsolve;        /* linear solves this step */
+  int iters[16];     /* Krylov iterations of each solve */
+  int used[16];      /* solver that finished each solve (11 -> 2 on fallback) */
+  double resid[16];  /* final recurrence residual of each solve */
+} orc_step_info;
+
+/* ---- element (Elmlib.f, Elmlib03.f) ---- */
+int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *xl,
+               const double *ul, const double *dpl, double *p, double *s, int isw,
+               const orc_env *env);
+double orc_tload(int nc, double t, const double *x, double u, double *vc, const orc_env *env);
+
+/* ---- structure (Adjacency.f, Rdata.f, Graph.f, Reord.f, Numeq.f, Csr.f) ---- */
+int orc_promote_hexa20(int *ix, int numel, int nnodev);
+void orc_bound_quad(int *id, const int *ix, int numel);
+int64_t orc_graph_count(const int *ix, int numel, int nnode);
+void orc_graph(const int *ix, int numel, int nnode, int64_t *ip, int *ips, int *work);
+void orc_genrcm(int n, const int64_t *xadj, const int *adj, int *perm);
+int orc_numeq(int *id, const int *num, const int *fnno, int nnode);
+int64_t orc_csrc_count(const int *id, const int *num, const int64_t *ip, const int *ips,
+                       int nnode, int neq, int64_t *ia);
+void orc_csrc_fill(const int *id, const int *num, const int64_t *ip, const int *ips, int nnode,
+                   int neq, const int64_t *ia, int *ja);
+
+/* ---- element loop + assembly (Pform.f, Assbly.f) ---- */
+int orc_pform(orc_model *md, int isw, int lhs, int rhs, double *bvec, const orc_env *env);
+void orc_porosity(orc_model *md, double *dporo, int *ic, const orc_env *env);
+
+/* ---- solvers (Matvec.f, Iters.f, Precond.f, Solv.f) ---- */
+void orc_matvec_csrc(int neq, const int64_t *ia, const int *ja, const double *ad,
+                     const double *al, const double *x, double *y);
+double orc_dot(const double *a, const double *b, int n);
+void orc_precond(int precond, int neq, const double *ad, double *m);
+int orc_pcg(int neq, const int64_t *ia, const int *ja, const double *ad, const double *al,
+            const double *m, const double *b, double *x, double tol, int maxit, double *res);
+int orc_rpsqmr(int neq, const int64_t *ia, const int *ja, const double *ad, const double *al,
+               const double *m, const double *b, double *x, double tol, int maxit,
+               double *res, int *fail);
+int orc_pbicgstab(int neq, const int64_t *ia, const int *ja, const double *ad,
+                  const double *al, const double *m, const double *b, double *x, double tol,
+                  int maxit, double *res);
+int orc_pminres(int neq, const int64_t *ia, const int *ja, const double *ad, const double *al,
+                const double *m, const double *b, double *x, double tol, int maxit,
+                int nrestart, double *res);
+int orc_gmres2(int neq, const int64_t *ia, const int *ja, const double *ad, const double *al,
+               const double *m, const double *b, double *x, int k, double tol, int maxit,
+               double *res);
+int orc_solv(int neq, const int64_t *ia, const int *ja, const double *ad, const double *al,
+             double *m, const double *b, double *x, const orc_cfg *cfg, int *used,
+             double *res);
+
+/* ---- time step (Pload.f, Predict.f, Residuo.f, Main.f) ---- */
+void orc_pload(orc_model *md, const orc_env *env);
+void orc_delta_predict(orc_model *md);
+void orc_delta_update(orc_model *md);
+void orc_update_res(orc_model *md);
+int orc_residuo(const orc_model *md, const double *b, int nli, double tol, double *state,
+                int crit);
+int orc_step(orc_model *md, const orc_cfg *cfg, orc_env *env, orc_step_info *info);
+
+#endif
