@@ -1,0 +1,177 @@
+/*
+ * poromec_hip.h -- C ABI of libpmhip, the MI355X (gfx950) implementation of
+ * poroMec's mef Biot u-p solve path: hexa20/8 element assembly, the sparse
+ * matrix-vector product, the Krylov solvers with their dot / axpy / Jacobi
+ * steps, and the backward-Euler time step around them.
+ *
+ * Plain pointers and sizes only.  Host arrays use the reference's layouts
+ * (Fortran column-major, 1-based node and equation numbers):
+ *   ix    (21, numel)  int32  element nodes 1..20 + material id (column 21)
+ *   x     (3, nnodev)  double vertex coordinates
+ *   id    (4, nnode)   int32  restraint flags in, equation numbers out
+ *   u     (4, nnode)   double u, v, w, p per node
+ *   e     (15, numat)  double material constants of elmt17_pm
+ *   ia    (neq+1)      int64  1-based row pointers of the strict lower CSRC
+ *   ja    (nad)        int32  1-based columns
+ *
+ * Every call returns PM_OK (0) or a negative PM_ERR_* code; pm_error_string
+ * names it.  A context is owned by one host thread at a time.
+ */
+#ifndef POROMEC_HIP_H
+#define POROMEC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pm_ctx pm_ctx;
+
+/* error codes: these replace the reference's stop_mef() exits */
+#define PM_OK 0
+#define PM_ERR_HIP (-1)          /* HIP runtime error (no device, out of memory, ...) */
+#define PM_ERR_ARG (-2)          /* bad argument or size */
+#define PM_ERR_ZERO_DIAG (-3)    /* zero diagonal coefficient (Iters.f:401-406) */
+#define PM_ERR_MAXIT (-4)        /* no convergence in maxit (Iters.f:499-501) */
+#define PM_ERR_BREAKDOWN (-5)    /* Krylov breakdown (Iters.f:2955-2991, 6260-6307) */
+#define PM_ERR_JACOBIAN (-6)     /* element Jacobian det <= 1e-14 (Elmlib.f:2363-2366) */
+#define PM_ERR_NLIT (-7)         /* nonlinear loop reached maxnlit (Main.f:1046-1053) */
+#define PM_ERR_STATE (-8)        /* call out of order (no mesh, no matrix, ...) */
+#define PM_ERR_UNSUPPORTED (-9)  /* feature outside this path (element type, load type) */
+
+/* solver ids (Solv.f dispatch) and preconditioner ids (Precond.f) */
+#define PM_SOLVER_PCG 1
+#define PM_SOLVER_GMRES 2
+#define PM_SOLVER_BICGSTAB 4
+#define PM_SOLVER_MINRES 7
+#define PM_SOLVER_SQMR 11
+#define PM_PRECOND_DIAG 2
+#define PM_PRECOND_ABSDIAG 5
+
+/* arithmetic of the Krylov solve */
+#define PM_PREC_FP64 0   /* everything in FP64 (the oracle form) */
+#define PM_PREC_FP32IR 1 /* FP32 matrix values and Jacobi inside FP64 iterative refinement */
+
+typedef struct {
+  int solver;      /* PM_SOLVER_* (reference default 11) */
+  int precond;     /* PM_PRECOND_* (reference default 2) */
+  int maxit;       /* Krylov iteration limit (reference default 50000) */
+  int nkrylov;     /* GMRES restart length ngram (reference default 150) */
+  double tol;      /* relative tolerance of the Krylov stop test */
+  int precision;   /* PM_PREC_* */
+  double ir_inner_tol; /* FP32IR: relative tolerance of each inner solve */
+  int ir_maxouter; /* FP32IR: refinement sweeps */
+  int maxnlit;     /* nonlinear iterations per step (nl macro, default 500) */
+  double nltol;    /* nonlinear tolerance (nl macro, default 1e-4) */
+  int stop_crit;   /* cal_residuo_pm criterion: 1 or 2 */
+} pm_solver_cfg;
+
+typedef struct {
+  int iters;        /* Krylov iterations (FP32IR: inner iterations, all sweeps) */
+  int outer;        /* FP32IR refinement sweeps (0 for FP64) */
+  int used_solver;  /* 2 when SQMR fell back to GMRES */
+  double resid;     /* final recurrence residual of the stop test */
+  double true_rel;  /* ||b - A x|| / ||b|| recomputed in FP64 */
+  double ms;        /* device time of the solve, milliseconds */
+  int spmv64, spmv32; /* SpMV launches by precision */
+} pm_solve_info;
+
+typedef struct {
+  int nlit;             /* nonlinear iterations */
+  int nsolve;           /* linear solves */
+  int iters[16];        /* Krylov iterations of each solve */
+  int used[16];         /* solver that finished each solve */
+  double true_rel[16];  /* FP64 relative residual of each solve */
+  double ms_total;      /* device time of the step */
+  double ms_solve;      /* device time inside the solves */
+  double ms_element;    /* device time inside element loops */
+} pm_step_info;
+
+const char *pm_error_string(int code);
+/* Text of the last HIP runtime error seen by this thread (PM_ERR_HIP). */
+const char *pm_last_error(void);
+
+/* Creates a context on HIP device `device` (one stream, default priority). */
+int pm_ctx_create(pm_ctx **ctx, int device);
+void pm_ctx_destroy(pm_ctx *ctx);
+
+/* ---- set-up: the mesh macro (Main.f:481-870) ----
+ * hexa8 input (ix columns 9..20 zero) is promoted to hexa20/8 as the deck
+ * reader does (Rdata.f:479-521, mk_elconn_quad_v1 Adjacency.f:583-693), the
+ * mid-edge restraints follow mk_bound_quad (Rdata.f:2337-2405), nodes are
+ * RCM-ordered (reord/genrcm_v1, Reord.f) when reorder != 0, equations are
+ * numbered by numeqpmec2 (Numeq.f:86-151), and the matrix pattern is built
+ * on the device.  Replaces rdat_pm's element block + reord + numeqpmec2 +
+ * datastruct_pm (Main.f:496-771). */
+int pm_mesh_setup(pm_ctx *ctx, int numel, int nnodev, const int *ix, const double *x,
+                  const int *idv, int reorder, int *nnode, int *neq, int64_t *nad);
+
+/* The same promotion, restraints, ordering and numbering on the host only
+ * (no device needed).  ix20 (21 x numel), id (4 x nnode_max) and perm
+ * (nnode_max) receive the results; nnode_max >= nnodev + 12 * numel always
+ * suffices.  nad is the size of the reference's strict-lower CSRC. */
+int pm_mesh_numbering(int numel, int nnodev, const int *ix, const int *idv, int reorder,
+                      int nnode_max, int *ix20, int *id, int *perm, int *nnode, int *neq,
+                      int64_t *nad);
+
+/* The promoted connectivity (21 x numel), equation numbers (4 x nnode) and
+ * the RCM permutation (nnode, perm[i] = node at position i+1), any may be NULL. */
+int pm_get_numbering(pm_ctx *ctx, int *ix20, int *id, int *perm);
+
+/* Materials, face loads, load table and gravity (materials / elmtloads /
+ * loads / gravity macros, Rdata.f:1132-1180, 1487, 3543).  Element type 17
+ * (elmt17_pm) only.  load: load(3,200), fload: fload(1000,5,200) of load.fi. */
+int pm_set_materials(pm_ctx *ctx, int numat, const double *e, const int *eload,
+                     const int *eloadp, const int *load, const double *fload,
+                     const double *gravity, double gravity_mod);
+
+/* Nodal loads and prescribed values: nload (4, nnode) time-function ids and
+ * f (4, nnode) values, as pload_pm reads them (Pload.f:1-92).  Optional. */
+int pm_set_nodal_loads(pm_ctx *ctx, const int *nload, const double *f);
+
+/* ---- element loop + assembly: pform_pm (Pform.f:1-429) with elmt17_pm
+ * (Elmlib03.f:1822-2636) and assbly_pm/csr (Assbly.f:89-304) ----
+ * isw 2: stiffness (lhs) and internal force b -= K_e u_e (rhs) from the
+ *        device state u; isw 4: b0 -= volume, face and previous-step terms.
+ * target 0 assembles the right-hand side into b0, 1 into b. */
+int pm_pform(pm_ctx *ctx, int isw, int lhs, int rhs, double dt, double t, int target);
+
+/* ---- linear solve: solv_pm (Solv.f:27-780) on the device matrix ----
+ * b: host right-hand side (neq) or NULL to use the device b; x: host
+ * solution (neq) or NULL to keep it on the device. */
+int pm_solve(pm_ctx *ctx, const pm_solver_cfg *cfg, const double *b, double *x,
+             pm_solve_info *info);
+
+/* ---- drop-in solv_pm with the caller's CSRC matrix ----
+ * Uploads the strict-lower CSRC pattern (ia, ja: Datastruct.f:124-150) and
+ * values (ad, al), then solves A x = b with A = diag(ad) + L + L^T.
+ * pm_struct_upload keeps the pattern; pm_values_upload replaces values. */
+int pm_struct_upload(pm_ctx *ctx, int neq, int64_t nad, const int64_t *ia, const int *ja);
+int pm_values_upload(pm_ctx *ctx, const double *ad, const double *al);
+
+/* y = A x (matvec_csrc_sym_pm, Matvec.f:1504-1595), host vectors of neq. */
+int pm_spmv(pm_ctx *ctx, const double *x, double *y);
+
+/* The device matrix as the reference's CSRC: ia (neq+1), ja (nad), ad (neq),
+ * al (nad); any may be NULL. */
+int pm_export_csrc(pm_ctx *ctx, int64_t *ia, int *ja, double *ad, double *al);
+
+/* ---- time march: the solv macro (Main.f:876-1123) ----
+ * State arrays: u, u0 (4, nnode), dp, porosity (nnode), time t. */
+int pm_set_state(pm_ctx *ctx, const double *u, const double *u0, const double *dp,
+                 const double *porosity, double t);
+int pm_get_state(pm_ctx *ctx, double *u, double *u0, double *dp, double *porosity,
+                 double *t);
+/* Device vectors b0, b, x0 (neq), any may be NULL. */
+int pm_get_vectors(pm_ctx *ctx, double *b0, double *b, double *x0);
+int pm_set_vectors(pm_ctx *ctx, const double *b0, const double *b, const double *x0);
+/* One backward-Euler step of size dt: pload_pm, pform isw=4, predictor,
+ * nonlinear loop (pform isw=2, cal_residuo_pm, solv_pm, delta_update_pm),
+ * porosity_form, update_res_v2. */
+int pm_step(pm_ctx *ctx, double dt, const pm_solver_cfg *cfg, pm_step_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

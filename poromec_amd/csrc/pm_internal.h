@@ -1,0 +1,174 @@
+// pm_internal.h -- internal definitions of libpmhip (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/poromec_hip.h"
+
+#define PM_NEN 20
+#define PM_NENV 8
+#define PM_NDF 4
+#define PM_NST 68
+#define PM_PROP 15
+#define PM_NUMLOAD 200
+#define PM_NTERMS 5
+#define PM_NPAR 1000
+
+#define PM_CHECK_HIP(expr)                                  \
+  do {                                                      \
+    hipError_t e__ = (expr);                                \
+    if (e__ != hipSuccess) {                                \
+      pm_set_last_error(hipGetErrorString(e__), __FILE__, __LINE__); \
+      return PM_ERR_HIP;                                    \
+    }                                                       \
+  } while (0)
+
+#define PM_TRY(expr)             \
+  do {                           \
+    int rc__ = (expr);           \
+    if (rc__ != PM_OK) return rc__; \
+  } while (0)
+
+void pm_set_last_error(const char *msg, const char *file, int line);
+
+// ---------------------------------------------------------------------------
+// host structure (pm_structure.cpp)
+// ---------------------------------------------------------------------------
+struct PmHostMesh {
+  int numel = 0, nnodev = 0, nnode = 0, neq = 0;
+  int64_t nad = 0;          // strict-lower count (reference CSRC size)
+  std::vector<int> ix20;    // (21, numel) 1-based, reference layout
+  std::vector<int> idflags; // (4, nnode) restraint flags
+  std::vector<int> id;      // (4, nnode) reference equation numbers (1-based, 0 none)
+  std::vector<int> perm;    // (nnode) RCM order, 1-based node ids
+  std::vector<int> fnno;    // (nnode) 1 for vertices
+  // node graph (graph_v1): 0-based CSR over nodes, neighbours sorted by id
+  std::vector<int64_t> gptr;
+  std::vector<int> gadj;
+  // element colouring
+  std::vector<int> colour_elems;  // elements grouped by colour (0-based)
+  std::vector<int> colour_ptr;    // ncolour+1
+};
+
+// Node-shared-column matrix layout ("node blocks"): every free node k (in
+// equation order) owns nr[k] consecutive rows starting at fr[k]; all its rows
+// share the column list ncol[colptr[k] .. colptr[k+1]) and row q's values
+// sit at val[valptr[k] + q * c_k + j], c_k = colptr[k+1] - colptr[k].
+struct PmHostPattern {
+  int nrn = 0;                 // free nodes (row blocks)
+  int neq = 0;
+  std::vector<int> fr, nr;     // per row block
+  std::vector<int64_t> colptr; // nrn+1
+  std::vector<int64_t> valptr; // nrn+1
+  std::vector<int> ncol;       // 0-based equation ids
+  std::vector<int> eq_block;   // per equation: its row block
+};
+
+int pm_host_promote(PmHostMesh &m, const int *ix8);
+void pm_host_bound_quad(PmHostMesh &m);
+void pm_host_graph(PmHostMesh &m);
+void pm_host_rcm(PmHostMesh &m, int reorder);
+void pm_host_numeq(PmHostMesh &m);
+int pm_host_pattern(const PmHostMesh &m, PmHostPattern &p);
+int pm_host_pattern_from_csrc(int neq, int64_t nad, const int64_t *ia, const int *ja,
+                              PmHostPattern &p);
+void pm_host_colour(PmHostMesh &m);
+int64_t pm_host_count_lower(const PmHostPattern &p);
+
+// ---------------------------------------------------------------------------
+// device context
+// ---------------------------------------------------------------------------
+struct PmScalars;  // device-side Krylov scalars (pm_krylov.hip)
+
+struct PmDevMatrix {
+  int nrn = 0, neq = 0;
+  int64_t nnz = 0, ncols = 0;
+  int *fr = nullptr, *nr = nullptr, *ncol = nullptr, *eq_block = nullptr;
+  int64_t *colptr = nullptr, *valptr = nullptr;
+  double *val = nullptr;
+  float *val32 = nullptr;
+  double *diag = nullptr;  // ad
+  int64_t *diagpos = nullptr; // position of the diagonal entry of each row
+  bool have32 = false;
+  bool values_valid = false;
+};
+
+struct pm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int cu_count = 256;
+
+  PmHostMesh mesh;
+  bool have_mesh = false;
+  bool have_materials = false;
+  PmDevMatrix A;
+  bool have_matrix = false;
+
+  // device mesh
+  int *d_ix = nullptr;      // (numel, 20) 0-based nodes
+  double *d_xv = nullptr;   // (nnodev, 3)
+  int *d_eq = nullptr;      // (nnode, 4) 0-based equation or -1
+  unsigned char *d_isp = nullptr; // (neq) 1 for pressure equations
+  int *d_elem_err = nullptr; // element error flag
+  std::vector<int64_t> csrc_ia; // caller's lower pattern (pm_struct_upload)
+  std::vector<int> csrc_ja;
+  int *d_mat = nullptr;     // (numel) 0-based material
+  double *d_e = nullptr;    // (numat, 15)
+  int numat = 0;
+  int *d_eload = nullptr;   // (numel, 14): mechanical 7, hydraulic 7
+  int has_face_loads = 0;
+  int *d_load = nullptr;    // (200, 3)
+  double *d_fload = nullptr;// (200, 5, 1000)
+  std::vector<int> h_load;
+  std::vector<double> h_fload;
+  double gravity[3] = {0, 0, 0};
+  double gravity_mod = 10.0;
+  int *d_colour_elems = nullptr;
+  std::vector<int> colour_ptr;
+  int *d_ic = nullptr;      // (nnode) elements incident to each vertex
+
+  // nodal loads
+  int *d_nload = nullptr;   // (nnode, 4)
+  double *d_f = nullptr;    // (nnode, 4)
+  bool have_nodal_loads = false;
+
+  // state
+  double *d_u = nullptr, *d_u0 = nullptr, *d_dp = nullptr, *d_poro = nullptr;
+  double *d_b0 = nullptr, *d_b = nullptr, *d_x0 = nullptr, *d_m = nullptr;
+  double *d_dporo = nullptr;
+  double t = 0.0;
+
+  // solver workspace
+  std::vector<void *> work;   // freed at destroy
+  double *d_wk[16] = {nullptr};
+  int wk_n = 0;               // length of each work vector
+  double *d_gm = nullptr;     // GMRES basis
+  int gm_k = 0;
+  double *d_partials = nullptr;
+  unsigned int *d_counter = nullptr;
+  PmScalars *d_scal = nullptr;
+  PmScalars *h_scal = nullptr; // pinned mirror
+  int red_blocks = 1024;
+
+  ~pm_ctx();
+};
+
+int pm_upload_pattern(pm_ctx *c, const PmHostPattern &p);
+int pm_dev_alloc(pm_ctx *c, void **ptr, size_t bytes);
+int pm_ensure_workspace(pm_ctx *c, int n);
+
+// kernels entry points (pm_blas.hip, pm_krylov.hip, pm_element.hip, pm_march.hip)
+int pm_spmv_dev(pm_ctx *c, const double *x, double *y, int use32);
+int pm_solve_dev(pm_ctx *c, const pm_solver_cfg *cfg, const double *b, double *x,
+                 pm_solve_info *info);
+int pm_pform_dev(pm_ctx *c, int isw, int lhs, int rhs, double dt, double t, double *bvec);
+int pm_porosity_dev(pm_ctx *c, double dt);
+int pm_extract_diag(pm_ctx *c);
+int pm_make_val32(pm_ctx *c);
+int pm_step_dev(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_info *info);
+double pm_dot_host(pm_ctx *c, const double *a, const double *b, int n, int *rc);

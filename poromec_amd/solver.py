@@ -1,0 +1,229 @@
+"""Host side of the mef Biot u-p path on the MI355X: a thin Python mirror of
+the reference's driver around the libpmhip C ABI.
+
+The reference's operator interface for this path is the ``mesh`` macro
+(structure), ``pform_pm`` (element loop + assembly, ``Pform.f:1``),
+``solv_pm`` (``Solv.f:27``) and the ``solv`` macro time step
+(``Main.f:876-1123``).  :class:`MefSolver` exposes the same steps with the
+same argument meaning (``isw``, ``lhs``, ``rhs``, ``solver``, ``precond``,
+``tol``, ``maxit``, ``ngram``) and raises :class:`PmError` where the
+reference calls ``stop_mef``.  All compute runs in the HIP library; the
+arrays here are inputs and outputs only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .problem import NDF, NEN, Deck
+
+SOLVERS = {"pcg": 1, "gmres": 2, "bicgstab": 4, "minres": 7, "sqmr": 11}
+SOLVER_NAMES = {v: k for k, v in SOLVERS.items()}
+
+
+@dataclass
+class SolverConfig:
+    """The ``solver`` config file fields (``read_solver_config_pm``,
+    ``Rdata.f:3141``) plus the ``nl`` macro (``Main.f:2100-2121``)."""
+
+    solver: str = "sqmr"        # reference default solver 11 (Main.f:289)
+    precond: int = 2            # 2 diag, 5 |diag|
+    tol: float = 1.0e-10
+    maxit: int = 50000
+    ngram: int = 150
+    precision: str = "fp64"     # "fp64" or "fp32ir"
+    ir_inner_tol: float = 1.0e-6
+    ir_maxouter: int = 10
+    maxnlit: int = 500
+    nltol: float = 1.0e-4
+    stop_crit: int = 1
+
+    def c(self) -> L.SolverCfg:
+        return L.SolverCfg(SOLVERS[self.solver], self.precond, self.maxit, self.ngram, self.tol,
+                           0 if self.precision == "fp64" else 1, self.ir_inner_tol,
+                           self.ir_maxouter, self.maxnlit, self.nltol, self.stop_crit)
+
+
+def mesh_numbering(deck: Deck, reorder: bool = True):
+    """Promotion, restraints, RCM and numbering on the host (no device)."""
+    lib = L.lib()
+    ix = L.as_i32(deck.ix)
+    idv = L.as_i32(deck.idv)
+    cap = deck.nnodev + 12 * deck.numel
+    ix20 = np.zeros((deck.numel, NEN + 1), np.int32)
+    idm = np.zeros((cap, NDF), np.int32)
+    perm = np.zeros(cap, np.int32)
+    nnode, neq, nad = C.c_int(), C.c_int(), C.c_int64()
+    L.check(lib.pm_mesh_numbering(deck.numel, deck.nnodev, L.ptr(ix), L.ptr(idv), int(reorder),
+                                  cap, L.ptr(ix20), L.ptr(idm), L.ptr(perm), C.byref(nnode),
+                                  C.byref(neq), C.byref(nad)), "pm_mesh_numbering")
+    n = nnode.value
+    return dict(ix=ix20, id=idm[:n].copy(), perm=perm[:n].copy(), nnode=n, neq=neq.value,
+                nad=int(nad.value))
+
+
+class MefSolver:
+    """One problem resident on one MI355X."""
+
+    def __init__(self, deck: Deck, device: int = 0, reorder: bool = True):
+        self.lib = L.lib()
+        self.deck = deck
+        self.ctx = C.c_void_p()
+        L.check(self.lib.pm_ctx_create(C.byref(self.ctx), device), "pm_ctx_create")
+        ix = L.as_i32(deck.ix)
+        x = L.as_f64(deck.x)
+        idv = L.as_i32(deck.idv)
+        nnode, neq, nad = C.c_int(), C.c_int(), C.c_int64()
+        L.check(self.lib.pm_mesh_setup(self.ctx, deck.numel, deck.nnodev, L.ptr(ix), L.ptr(x),
+                                       L.ptr(idv), int(reorder), C.byref(nnode), C.byref(neq),
+                                       C.byref(nad)), "pm_mesh_setup")
+        self.nnode, self.neq, self.nad = nnode.value, neq.value, int(nad.value)
+        self.set_materials(deck)
+        self.t = 0.0
+
+    # -- set-up ------------------------------------------------------------
+    def set_materials(self, deck: Deck):
+        e = L.as_f64(deck.e)
+        L.check(self.lib.pm_set_materials(self.ctx, e.shape[0], L.ptr(e), L.ptr(L.as_i32(deck.eload)),
+                                          L.ptr(L.as_i32(deck.eloadp)), L.ptr(L.as_i32(deck.load)),
+                                          L.ptr(L.as_f64(deck.fload)),
+                                          L.ptr(L.as_f64(deck.gravity)), float(deck.gravity_mod)),
+                "pm_set_materials")
+
+    def numbering(self):
+        ix20 = np.zeros((self.deck.numel, NEN + 1), np.int32)
+        idm = np.zeros((self.nnode, NDF), np.int32)
+        perm = np.zeros(self.nnode, np.int32)
+        L.check(self.lib.pm_get_numbering(self.ctx, L.ptr(ix20), L.ptr(idm), L.ptr(perm)),
+                "pm_get_numbering")
+        return ix20, idm, perm
+
+    # -- pform_pm / solv_pm --------------------------------------------------
+    def pform(self, isw: int, lhs: bool, rhs: bool, dt: float, t: float, target: str = "b"):
+        L.check(self.lib.pm_pform(self.ctx, isw, int(lhs), int(rhs), dt, t,
+                                  1 if target == "b" else 0), "pm_pform")
+
+    def solve(self, b=None, cfg: SolverConfig | None = None, return_x: bool = True):
+        cfg = cfg or SolverConfig()
+        info = L.SolveInfo()
+        bb = None if b is None else L.as_f64(b)
+        x = np.zeros(self.neq) if return_x else None
+        L.check(self.lib.pm_solve(self.ctx, C.byref(cfg.c()), L.ptr(bb), L.ptr(x),
+                                  C.byref(info)), "pm_solve")
+        return x, _solve_info(info)
+
+    def spmv(self, x):
+        y = np.zeros(self.neq)
+        L.check(self.lib.pm_spmv(self.ctx, L.ptr(L.as_f64(x)), L.ptr(y)), "pm_spmv")
+        return y
+
+    def export_csrc(self):
+        ia = np.zeros(self.neq + 1, np.int64)
+        L.check(self.lib.pm_export_csrc(self.ctx, L.ptr(ia), None, None, None), "pm_export_csrc")
+        nad = int(ia[-1] - 1)
+        ja = np.zeros(max(nad, 1), np.int32)
+        al = np.zeros(max(nad, 1))
+        ad = np.zeros(self.neq)
+        L.check(self.lib.pm_export_csrc(self.ctx, L.ptr(ia), L.ptr(ja), L.ptr(ad), L.ptr(al)),
+                "pm_export_csrc")
+        return ia, ja[:nad], ad, al[:nad]
+
+    # -- state and the time step ------------------------------------------
+    def set_state(self, u=None, u0=None, dp=None, porosity=None, t=0.0):
+        L.check(self.lib.pm_set_state(self.ctx, L.ptr(None if u is None else L.as_f64(u)),
+                                      L.ptr(None if u0 is None else L.as_f64(u0)),
+                                      L.ptr(None if dp is None else L.as_f64(dp)),
+                                      L.ptr(None if porosity is None else L.as_f64(porosity)),
+                                      float(t)), "pm_set_state")
+        self.t = t
+
+    def state(self):
+        u = np.zeros((self.nnode, NDF))
+        u0 = np.zeros((self.nnode, NDF))
+        dp = np.zeros(self.nnode)
+        por = np.zeros(self.nnode)
+        t = C.c_double()
+        L.check(self.lib.pm_get_state(self.ctx, L.ptr(u), L.ptr(u0), L.ptr(dp), L.ptr(por),
+                                      C.byref(t)), "pm_get_state")
+        return dict(u=u, u0=u0, dp=dp, porosity=por, t=t.value)
+
+    def vectors(self):
+        b0, b, x0 = np.zeros(self.neq), np.zeros(self.neq), np.zeros(self.neq)
+        L.check(self.lib.pm_get_vectors(self.ctx, L.ptr(b0), L.ptr(b), L.ptr(x0)), "pm_get_vectors")
+        return b0, b, x0
+
+    def set_vectors(self, b0=None, b=None, x0=None):
+        f = lambda a: None if a is None else L.ptr(L.as_f64(a))  # noqa: E731
+        L.check(self.lib.pm_set_vectors(self.ctx, f(b0), f(b), f(x0)), "pm_set_vectors")
+
+    def step(self, cfg: SolverConfig | None = None, dt: float | None = None):
+        cfg = cfg or SolverConfig()
+        info = L.StepInfo()
+        dt = self.deck.dt if dt is None else dt
+        L.check(self.lib.pm_step(self.ctx, dt, C.byref(cfg.c()), C.byref(info)), "pm_step")
+        self.t += dt
+        n = min(info.nsolve, 16)
+        return dict(nlit=info.nlit, nsolve=info.nsolve, iters=list(info.iters[:n]),
+                    used=list(info.used[:n]), true_rel=list(info.true_rel[:n]),
+                    ms_total=info.ms_total, ms_solve=info.ms_solve, ms_element=info.ms_element)
+
+    def close(self):
+        if self.ctx:
+            self.lib.pm_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CsrcSolver:
+    """Drop-in ``solv_pm`` for a caller that assembles its own CSRC matrix."""
+
+    def __init__(self, ia, ja, device: int = 0):
+        self.lib = L.lib()
+        self.ctx = C.c_void_p()
+        L.check(self.lib.pm_ctx_create(C.byref(self.ctx), device), "pm_ctx_create")
+        self.ia = np.ascontiguousarray(ia, np.int64)
+        self.ja = L.as_i32(ja)
+        self.neq = len(self.ia) - 1
+        L.check(self.lib.pm_struct_upload(self.ctx, self.neq, int(self.ia[-1] - 1),
+                                          L.ptr(self.ia), L.ptr(self.ja)), "pm_struct_upload")
+
+    def set_values(self, ad, al):
+        L.check(self.lib.pm_values_upload(self.ctx, L.ptr(L.as_f64(ad)), L.ptr(L.as_f64(al))),
+                "pm_values_upload")
+
+    def solve(self, b, cfg: SolverConfig | None = None):
+        cfg = cfg or SolverConfig()
+        info = L.SolveInfo()
+        x = np.zeros(self.neq)
+        L.check(self.lib.pm_solve(self.ctx, C.byref(cfg.c()), L.ptr(L.as_f64(b)), L.ptr(x),
+                                  C.byref(info)), "pm_solve")
+        return x, _solve_info(info)
+
+    def spmv(self, x):
+        y = np.zeros(self.neq)
+        L.check(self.lib.pm_spmv(self.ctx, L.ptr(L.as_f64(x)), L.ptr(y)), "pm_spmv")
+        return y
+
+    def close(self):
+        if self.ctx:
+            self.lib.pm_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _solve_info(info: L.SolveInfo) -> dict:
+    return dict(iters=info.iters, outer=info.outer, used=info.used_solver, resid=info.resid,
+                true_rel=info.true_rel, ms=info.ms, spmv64=info.spmv64, spmv32=info.spmv32)

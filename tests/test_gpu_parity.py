@@ -1,0 +1,190 @@
+"""GPU parity of the HIP path (through the C ABI) against the oracle.
+
+Bars (from the task statement of this path):
+* CSRC sparsity bit-exact (ia, ja);
+* matrix values within 1e-13 relative (|a_gpu - a_orc| <= 1e-13 * max|a| of
+  the row: the GPU contracts FMAs and sums element contributions in colour
+  order, so entries that cancel to ~0 are compared on the row's scale);
+* right-hand sides within 1e-13 of max|b|;
+* Krylov iteration counts within +-2 of the oracle's, solutions within 1e-9;
+* displacement and pressure histories within 1e-9 relative at every step.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from poromec_amd import problem as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(deck, **kw):
+    from poromec_amd.solver import MefSolver
+    return MefSolver(deck, **kw)
+
+
+def _rowwise_close(ia, ad_g, al_g, ad_o, al_o, rtol=1e-13):
+    neq = len(ad_o)
+    worst = 0.0
+    for r in range(neq):
+        s, e = ia[r] - 1, ia[r + 1] - 1
+        scale = max(abs(ad_o[r]), np.abs(al_o[s:e]).max() if e > s else 0.0)
+        err = max(abs(ad_g[r] - ad_o[r]), np.abs(al_g[s:e] - al_o[s:e]).max() if e > s else 0.0)
+        worst = max(worst, err / scale if scale > 0 else err)
+    return worst
+
+
+@pytest.fixture(scope="module")
+def column():
+    d = P.column_deck(2, 2, 6, dt=0.01, nsteps=6)
+    st = O.build_structure(d)
+    return d, st
+
+
+def test_csrc_pattern_and_values(column):
+    d, st = column
+    s = _solver(d)
+    s.pform(2, True, True, d.dt, d.dt)
+    ia, ja, ad, al = s.export_csrc()
+    assert np.array_equal(ia, st["ia"])
+    assert np.array_equal(ja, st["ja"])
+    m = O.OracleModel(d, st)
+    m.assemble(isw=2, lhs=True, rhs=True)
+    assert _rowwise_close(st["ia"], ad, al, m.ad, m.al[: st["nad"]]) < 1e-13
+
+
+def test_internal_force_and_rhs(column):
+    d, st = column
+    rng = np.random.default_rng(11)
+    s = _solver(d)
+    m = O.OracleModel(d, st)
+    u = rng.standard_normal((st["nnode"], 4)) * 1e-3
+    dp = rng.standard_normal(st["nnode"]) * 1e-4
+    u[d.nnodev:, 3] = 0.0
+    dp[d.nnodev:] = 0.0
+    m.u[:] = u
+    m.dp[:] = dp
+    s.set_state(u=u, u0=np.zeros_like(u), dp=dp, t=0.0)
+    for isw in (2, 4):
+        m.b[:] = 0.0
+        m.env.t = 0.5
+        m.assemble(isw=isw, lhs=(isw == 2), rhs=True)
+        s.set_vectors(b=np.zeros(st["neq"]))
+        s.pform(isw, isw == 2, True, d.dt, 0.5, target="b")
+        _, b, _ = s.vectors()
+        scale = np.abs(m.b).max()
+        assert np.abs(b - m.b).max() <= 1e-13 * scale, isw
+
+
+def test_spmv(column):
+    d, st = column
+    s = _solver(d)
+    s.pform(2, True, False, d.dt, d.dt)
+    m = O.OracleModel(d, st)
+    m.assemble(isw=2, lhs=True, rhs=False)
+    x = np.random.default_rng(2).standard_normal(st["neq"])
+    y_o = np.zeros(st["neq"])
+    O.lib.orc_matvec_csrc(st["neq"], st["ia"], st["ja"], m.ad, m.al, x, y_o)
+    y = s.spmv(x)
+    assert np.abs(y - y_o).max() <= 1e-13 * np.abs(y_o).max()
+
+
+@pytest.mark.parametrize("solver,precond", [("sqmr", 2), ("pcg", 2), ("bicgstab", 2),
+                                            ("minres", 5), ("gmres", 2)])
+def test_solvers_vs_oracle(column, solver, precond):
+    from poromec_amd.solver import SolverConfig
+    d, st = column
+    s = _solver(d)
+    s.pform(2, True, False, d.dt, d.dt)
+    m = O.OracleModel(d, st)
+    m.assemble(isw=2, lhs=True, rhs=False)
+    neq = st["neq"]
+    b = np.random.default_rng(4).standard_normal(neq)
+    mvec = np.zeros(neq)
+    O.lib.orc_precond(precond, neq, m.ad, mvec)
+    x_o = np.zeros(neq)
+    fn = {"sqmr": O.lib.orc_rpsqmr, "pcg": O.lib.orc_pcg, "bicgstab": O.lib.orc_pbicgstab,
+          "minres": O.lib.orc_pminres, "gmres": O.lib.orc_gmres2}[solver]
+    import ctypes as C
+    res = C.c_double()
+    args = [neq, st["ia"], st["ja"], m.ad, m.al, mvec, b, x_o]
+    if solver == "sqmr":
+        fail = C.c_int()
+        it_o = fn(*args, 1e-10, 20000, C.byref(res), C.byref(fail))
+    elif solver == "minres":
+        it_o = fn(*args, 1e-10, 20000, 10, C.byref(res))
+    elif solver == "gmres":
+        it_o = fn(*args, 150, 1e-10, 200, C.byref(res))
+    else:
+        it_o = fn(*args, 1e-10, 20000, C.byref(res))
+    if it_o < 0:
+        pytest.skip(f"oracle {solver} does not converge on this indefinite system ({it_o})")
+    cfg = SolverConfig(solver=solver, precond=precond, tol=1e-10, maxit=20000 if solver != "gmres" else 200)
+    x, info = s.solve(b, cfg)
+    assert abs(info["iters"] - it_o) <= 2, (info["iters"], it_o)
+    assert np.abs(x - x_o).max() <= 1e-9 * np.abs(x_o).max()
+
+
+def test_fp32ir_reaches_tolerance(column):
+    from poromec_amd.solver import SolverConfig
+    d, st = column
+    s = _solver(d)
+    s.pform(2, True, False, d.dt, d.dt)
+    b = np.random.default_rng(5).standard_normal(st["neq"])
+    x64, i64 = s.solve(b, SolverConfig(tol=1e-10))
+    x32, i32 = s.solve(b, SolverConfig(tol=1e-10, precision="fp32ir"))
+    assert i32["true_rel"] < 1e-10
+    assert i32["spmv32"] > 0
+    assert np.abs(x32 - x64).max() <= 1e-8 * np.abs(x64).max()
+
+
+def test_csrc_dropin_solve(column):
+    from poromec_amd.solver import CsrcSolver, SolverConfig
+    d, st = column
+    m = O.OracleModel(d, st)
+    m.assemble(isw=2, lhs=True, rhs=False)
+    cs = CsrcSolver(st["ia"], st["ja"])
+    cs.set_values(m.ad, m.al[: st["nad"]])
+    x = np.random.default_rng(9).standard_normal(st["neq"])
+    y_o = np.zeros(st["neq"])
+    O.lib.orc_matvec_csrc(st["neq"], st["ia"], st["ja"], m.ad, m.al, x, y_o)
+    assert np.abs(cs.spmv(x) - y_o).max() <= 1e-14 * np.abs(y_o).max()
+    xs, info = cs.solve(y_o, SolverConfig(tol=1e-10))
+    assert info["true_rel"] < 1e-9
+
+
+def _history_errors(d, nsteps, cfg_kw=None, oracle_kw=None):
+    from poromec_amd.solver import SolverConfig
+    st = O.build_structure(d)
+    m = O.OracleModel(d, st, **(oracle_kw or {}))
+    s = _solver(d)
+    cfg = SolverConfig(**(cfg_kw or {}))
+    worst_u = worst_p = 0.0
+    its = []
+    for k in range(nsteps):
+        io = m.step()
+        ig = s.step(cfg)
+        g = s.state()
+        nv = d.nnodev
+        eu = np.abs(g["u"][:, :3] - m.u[:, :3]).max() / max(np.abs(m.u[:, :3]).max(), 1e-300)
+        ep = np.abs(g["u"][:nv, 3] - m.u[:nv, 3]).max() / max(np.abs(m.u[:nv, 3]).max(), 1e-300)
+        worst_u, worst_p = max(worst_u, eu), max(worst_p, ep)
+        its.append((io["iters"], ig["iters"]))
+        assert io["nlit"] == ig["nlit"], (k, io, ig)
+        for a, b in zip(io["iters"], ig["iters"]):
+            assert abs(a - b) <= 2, (k, a, b)
+        dpe = np.abs(g["dp"][:nv] - m.dp[:nv]).max() / max(np.abs(m.dp[:nv]).max(), 1e-300)
+        assert dpe < 1e-9, (k, dpe)
+    return worst_u, worst_p, its
+
+
+def test_time_march_histories_column():
+    d = P.column_deck(2, 2, 8, dt=0.01, nsteps=8)
+    eu, ep, its = _history_errors(d, 8)
+    assert eu < 1e-9 and ep < 1e-9, (eu, ep, its)
+
+
+def test_time_march_histories_heterogeneous():
+    d = P.heterogeneous_box_deck(4, seed=7, dt=0.1, nsteps=4, corr_len=2.0)
+    eu, ep, its = _history_errors(d, 4)
+    assert eu < 1e-9 and ep < 1e-9, (eu, ep, its)
