@@ -1,0 +1,154 @@
+"""Pins the oracle's assembly and time-step bookkeeping.
+
+* csr (Assbly.f:218-304), pload_pm (Pload.f:1-92), delta_predict_pm,
+  delta_update_pm (Predict.f:117-215) and update_res_v2 (Pload.f:483-541):
+  bit for bit against the reference's own Fortran routines (oracle/_ref);
+* the assembled CSRC equals the sum of element matrices (dense brute force);
+* the whole time march against Terzaghi's one-dimensional consolidation
+  solution (restated from the theory, as in python/Terzaghi/Terzaghi.py of
+  the reference): undrained pore pressure p0 = alpha M q / (K_v + alpha^2 M),
+  p(z', t) = p0 sum_m 4/(m pi) sin(m pi z'/2H) exp(-m^2 pi^2 T/4), m odd,
+  T = c_v t / H^2, c_v = (k/mu) / (1/M + alpha^2/K_v), z' depth below the
+  drained top, to discretisation accuracy.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from poromec_amd import problem as P
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def test_assembly_csr_bitwise(ref):
+    d = P.column_deck(2, 1, 3)
+    m = O.OracleModel(d)
+    st = m.st
+    rng = np.random.default_rng(0)
+    m.u[:] = rng.standard_normal(m.u.shape) * 1e-3
+    m.assemble(isw=2, lhs=True, rhs=True)
+    ad_r = np.zeros(st["neq"])
+    al_r = np.zeros(st["nad"])
+    b_r = np.zeros(st["neq"])
+    for e in range(d.numel):
+        nodes = st["ix"][e, :20] - 1
+        ld = np.concatenate([st["id"][nodes, :3].ravel(), st["id"][nodes[:8], 3]]).astype(np.int32)
+        ul = np.concatenate([m.u[nodes, :3].ravel(), m.u[nodes[:8], 3]])
+        xl = d.x[nodes[:8]]
+        z7 = np.zeros(7, np.int32)
+        _, p, s = O.elmt17(d.e[0], z7, z7, xl, ul, np.zeros(8), 2, m.env)
+        sf = np.asfortranarray(s)
+        ints = [C.c_int(68), C.c_int(st["neq"])]
+        flags = [C.c_int(1), C.c_int(1), C.c_int(0)]
+        ref.csr_(_p(sf), _p(p), _p(ld), _p(st["ia"]), _p(st["ja"]), _p(al_r), _p(al_r), _p(ad_r),
+                 _p(b_r), C.byref(ints[0]), C.byref(ints[1]), *[C.byref(f) for f in flags])
+    assert np.array_equal(ad_r, m.ad)
+    assert np.array_equal(al_r, m.al[: st["nad"]])
+    assert np.array_equal(b_r, m.b)
+
+
+def test_assembly_equals_sum_of_element_matrices():
+    d = P.column_deck(1, 2, 2)
+    m = O.OracleModel(d)
+    st = m.st
+    m.assemble(isw=2, lhs=True, rhs=False)
+    n = st["neq"]
+    K = np.zeros((n + 1, n + 1))
+    for e in range(d.numel):
+        nodes = st["ix"][e, :20] - 1
+        ld = np.concatenate([st["id"][nodes, :3].ravel(), st["id"][nodes[:8], 3]])
+        z7 = np.zeros(7, np.int32)
+        _, _, s = O.elmt17(d.e[0], z7, z7, d.x[nodes[:8]], np.zeros(68), np.zeros(8), 2, m.env)
+        K[np.ix_(ld, ld)] += s  # equation 0 collects restrained dofs
+    K = K[1:, 1:]
+    assert np.allclose(np.diag(K), m.ad, rtol=1e-13, atol=0)
+    for i in range(n):
+        for k in range(st["ia"][i] - 1, st["ia"][i + 1] - 1):
+            j = st["ja"][k] - 1
+            assert abs(K[i, j] - m.al[k]) <= 1e-13 * abs(K[i]).max()
+        # nothing outside the pattern
+        lower = set(st["ja"][st["ia"][i] - 1: st["ia"][i + 1] - 1] - 1)
+        for j in np.nonzero(K[i, :i])[0]:
+            assert j in lower
+
+
+def test_step_bookkeeping_bitwise(ref):
+    d = P.column_deck(1, 1, 3)
+    st = O.build_structure(d)
+    nnode, neq = st["nnode"], st["neq"]
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal((nnode, 4))
+    u0 = rng.standard_normal((nnode, 4))
+    dp = rng.standard_normal(nnode)
+    x0 = rng.standard_normal(neq)
+    load, fload = P.empty_load_table()
+    load[0] = (2, 1, 0)
+    fload[0, 0, 0], fload[0, 0, 1] = 0.3, 0.7  # c = 0.3 + 0.7 t
+    nload = np.zeros((nnode, 4), np.int32)
+    nload[rng.random((nnode, 4)) < 0.3] = 1
+    f = rng.standard_normal((nnode, 4))
+    m = O.OracleModel(d, st)
+    m.env = O.make_env(load, fload, 0.05, 0.4)
+    m.nload[:] = nload
+    m.f[:] = f
+    m.u[:] = u
+    m.u0[:] = u0
+    m.dp[:] = dp
+    m.x0[:] = x0
+    ur, fr_, b0r = u.copy(), f.copy(), np.zeros(neq)
+    O.ref_set_common(O.load_ref(), 0.05, 0.4, (0, 0, 0), 10.0, load, fload)
+    nn, ndf, ndm = C.c_int(nnode), C.c_int(4), C.c_int(3)
+    xfull = np.zeros((nnode, 3))
+    xfull[: d.nnodev] = d.x
+    ref.pload_pm_(_p(st["id"]), _p(xfull), _p(fr_), _p(ur), _p(b0r), _p(nload), _p(st["fnno"]),
+                  C.byref(nn), C.byref(ndf), C.byref(ndm))
+    O.lib.orc_pload(C.byref(m._md), C.byref(m.env))
+    assert np.array_equal(m.b0, b0r) and np.array_equal(m.u, ur) and np.array_equal(m.f, fr_)
+    u0r = u0.copy()
+    ref.delta_predict_pm_(C.byref(nn), C.byref(ndf), _p(ur), _p(u0r), _p(st["fnno"]))
+    O.lib.orc_delta_predict(C.byref(m._md))
+    assert np.array_equal(m.u, ur)
+    ref.delta_update_pm_(C.byref(nn), C.byref(ndf), _p(st["id"]), _p(ur), _p(x0), _p(st["fnno"]))
+    O.lib.orc_delta_update(C.byref(m._md))
+    assert np.array_equal(m.u, ur)
+    dpr = dp.copy()
+    ref.update_res_v2_(C.byref(nn), C.byref(ndf), _p(ur), _p(u0r), _p(dpr), _p(st["fnno"]))
+    O.lib.orc_update_res(C.byref(m._md))
+    assert np.array_equal(m.u, ur) and np.array_equal(m.u0, u0r) and np.array_equal(m.dp, dpr)
+
+
+def terzaghi(zp, H, t, cv, p0, nterms=200):
+    T = cv * t / H**2
+    p = np.zeros_like(zp, dtype=float)
+    for i in range(nterms):
+        mm = 2 * i + 1
+        p += 4.0 / (mm * np.pi) * np.sin(mm * np.pi * zp / (2 * H)) * np.exp(-mm * mm * np.pi**2 * T / 4)
+    return p0 * p
+
+
+def test_terzaghi_column():
+    E, nu, kmu, M, q, H = 10.0, 0.3, 0.1, 1000.0, 1.0e-3, 10.0
+    d = P.column_deck(1, 1, int(H), E=E, nu=nu, k=kmu * 1e-9, mu=1e-9, load_mpa=q, dt=0.25)
+    m = O.OracleModel(d)
+    Kv = E * (1 - nu) / ((1 + nu) * (1 - 2 * nu))
+    cv = kmu / (1.0 / M + 1.0 / Kv)
+    p0 = M * q / (Kv + M)
+    nv = d.nnodev
+    z = d.x[:, 2]
+    corner = (d.x[:, 0] == 0) & (d.x[:, 1] == 0)
+    worst = 0.0
+    for step in range(1, 61):
+        m.step()
+        p = m.u[:nv, 3][corner]
+        pa = terzaghi(H - z[corner], H, m.t, cv, p0)
+        if step == 1:  # undrained response at the impermeable base
+            base = z[corner] == 0
+            assert abs(p[base][0] / p0 - 1.0) < 1e-3
+        if step >= 12:  # t >= 3 s: the boundary layer at the drained top has spread
+            worst = max(worst, np.abs(p - pa).max() / p0)
+    # backward Euler (dt = 0.25 s) on 10 quadratic elements
+    assert worst < 0.015, worst
