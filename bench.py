@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the mef Biot u-p path on one MI355X.
+
+Workload: a box of n^3 hex8 cells (default 128^3) promoted to the reference's
+hexa20/8 element, with a seeded log-normal permeability field (sigma 1,
+correlation length 8 cells), E ~ U[5, 50] MPa and nu ~ U[0.25, 0.35] per
+element, a seeded 10 % patch of 1 kPa top load, dt = 0.1 s, SQMR with Jacobi
+to 1e-10.  A "step" is one backward-Euler time step of the solv macro.
+
+The FP64 form and the FP32+IR form (FP32 matrix values and Jacobi inside an
+FP64 iterative-refinement loop) each run W untimed steps, then exactly K
+timed steps; `value` is the FP32+IR form's time steps per second.  The SpMV
+kernel time is measured with HIP events on the library's stream.  The CPU
+baseline is the oracle (a scalar C restatement of the reference) timed on a
+small box of the same workload on this host and scaled to the benchmark box.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "time-steps/s at 8.6M-dof Biot u-p (FP32+IR vs FP64 @1e-10); SpMV GB/s"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spmv_bytes(info, value_bytes):
+    """Bytes one SpMV launch must move in the node-block layout: every value
+    once, every shared column index once, the per-block metadata (colptr,
+    valptr: 8 B each; first row, row count: 4 B each), x once, y once."""
+    return (value_bytes * info["nnz"] + 4 * info["ncols"] + 24 * info["nblocks"]
+            + 16 * info["neq"])
+
+
+def csr_bytes(info):
+    """Bytes of the same product as a plain full CSR (int32 columns, FP64
+    values, int64 row pointers, x once, y once)."""
+    return 12 * info["nnz"] + 8 * (info["neq"] + 1) + 16 * info["neq"]
+
+
+def run_form(solver, cfg, warmup, steps, keep_states=False):
+    from poromec_amd.solver import MefSolver  # noqa: F401
+
+    z4 = np.zeros((solver.nnode, 4))
+    zn = np.zeros(solver.nnode)
+    solver.set_state(u=z4, u0=z4, dp=zn, porosity=zn, t=0.0)
+    states, infos = [], []
+    for k in range(warmup):
+        info = solver.step(cfg)
+        log(f"  {cfg.precision} warmup step {k}: {info['ms_total']:.0f} ms, iters {info['iters']}")
+        if keep_states is not None:
+            st = solver.state()
+            states.append((st["u"].copy(), st["dp"].copy()))
+    t0 = time.perf_counter()
+    for k in range(steps):
+        info = solver.step(cfg)
+        infos.append(info)
+        log(f"  {cfg.precision} timed step {k}: {info['ms_total']:.0f} ms, iters {info['iters']}")
+        if keep_states is not None:
+            # state readback happens after the step returned; excluded below
+            pass
+    wall = time.perf_counter() - t0
+    if keep_states is not None and steps:
+        st = solver.state()
+        states.append((st["u"].copy(), st["dp"].copy()))
+    return wall, infos, states
+
+
+def cpu_baseline(n_sample, n_bench_numel, bench_nnz, gpu_iters_per_step, seed=0):
+    """The oracle timed on an n_sample^3 box of the same generator; scaled to
+    the benchmark box by solver work (iterations x stored entries) and
+    element work (elements x element passes)."""
+    from oracle import oracle as O
+    from poromec_amd import problem as P
+
+    deck = P.heterogeneous_box_deck(n_sample, seed=seed, dt=0.1, patch_fraction=0.1)
+    m = O.OracleModel(deck)
+    nnz_full = 2 * m.st["nad"] + m.st["neq"]
+    t0 = time.perf_counter()
+    m.assemble(isw=2, lhs=True, rhs=True)
+    t_elem = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    info = m.step()
+    t_step = time.perf_counter() - t0
+    iters = sum(info["iters"])
+    passes = 3  # isw 4, isw 2 (matrix), isw 2 (residual check); porosity is lighter
+    t_solve = max(t_step - passes * t_elem, 1e-9)
+    per_iter_nnz = t_solve / max(iters, 1) / nnz_full
+    per_elem_pass = t_elem / deck.numel
+    est = per_iter_nnz * bench_nnz * gpu_iters_per_step + per_elem_pass * n_bench_numel * passes
+    return {
+        "value": 1.0 / est,
+        "unit": "time-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle (scalar C restatement) on a {n_sample}^3 box of the same generator: "
+                   f"one step took {t_step:.2f} s ({iters} SQMR iterations, {nnz_full} stored "
+                   f"entries, element pass {t_elem:.2f} s); scaled to the benchmark box by "
+                   f"iterations x entries ({gpu_iters_per_step} iterations/step measured on the "
+                   f"GPU) and elements x 3 element passes"),
+        "sample_seconds": t_step + t_elem,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128, help="box edge in cells")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--spmv-reps", type=int, default=20)
+    ap.add_argument("--cpu-sample-n", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this path has no multi-GPU form "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from poromec_amd import problem as P
+    from poromec_amd.solver import MefSolver, SolverConfig
+
+    n = args.n
+    t0 = time.perf_counter()
+    deck = P.heterogeneous_box_deck(n, seed=args.seed, dt=0.1, patch_fraction=0.1, nsteps=50)
+    log(f"deck {n}^3: {deck.numel} elements, {deck.nnodev} vertices ({time.perf_counter()-t0:.1f} s)")
+    t0 = time.perf_counter()
+    solver = MefSolver(deck)
+    setup_s = time.perf_counter() - t0
+    info = solver.matrix_info()
+    log(f"setup {setup_s:.1f} s: nnode {solver.nnode}, neq {solver.neq}, nnz {info['nnz']}, "
+        f"blocks {info['nblocks']}")
+
+    cfg64 = SolverConfig(precision="fp64", tol=1e-10)
+    cfg32 = SolverConfig(precision="fp32ir", tol=1e-10, ir_inner_tol=1e-6)
+
+    wall64, infos64, st64 = run_form(solver, cfg64, args.warmup, args.steps, keep_states=True)
+    wall32, infos32, st32 = run_form(solver, cfg32, args.warmup, args.steps, keep_states=True)
+
+    # error of the FP32+IR histories against the FP64 form, step by step
+    nv = deck.nnodev
+    err_u = err_p = 0.0
+    for (u64, _), (u32, _) in zip(st64, st32):
+        su = max(np.abs(u64[:, :3]).max(), 1e-300)
+        sp = max(np.abs(u64[:nv, 3]).max(), 1e-300)
+        err_u = max(err_u, np.abs(u32[:, :3] - u64[:, :3]).max() / su)
+        err_p = max(err_p, np.abs(u32[:nv, 3] - u64[:nv, 3]).max() / sp)
+
+    ms64 = solver.bench_spmv(False, args.spmv_reps)
+    ms32 = solver.bench_spmv(True, args.spmv_reps)
+    b64, b32 = spmv_bytes(info, 8), spmv_bytes(info, 4)
+    gbps64, gbps32 = b64 / ms64 / 1e6, b32 / ms32 / 1e6
+    log(f"SpMV fp64 {ms64:.3f} ms ({gbps64:.0f} GB/s), fp32 {ms32:.3f} ms ({gbps32:.0f} GB/s)")
+
+    steps = args.steps
+    it64 = sum(sum(i["iters"]) for i in infos64) / max(steps, 1)
+    it32 = sum(sum(i["iters"]) for i in infos32) / max(steps, 1)
+    ms_step64 = 1e3 * wall64 / max(steps, 1)
+    ms_step32 = 1e3 * wall32 / max(steps, 1)
+    rel64 = max((r for i in infos64 for r in i["true_rel"]), default=0.0)
+    rel32 = max((r for i in infos32 for r in i["true_rel"]), default=0.0)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.cpu_sample_n, deck.numel, info["nnz"], int(round(it64)))
+            log(f"cpu baseline: {cpu['value']:.3e} steps/s ({cpu['sample_seconds']:.1f} s of CPU work)")
+        except Exception as exc:  # the oracle is optional on the benchmark host
+            log(f"cpu baseline skipped: {exc}")
+
+    out = {
+        "metric": METRIC,
+        "value": 1e3 / ms_step32,
+        "unit": "time-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-spmv/f64-ir",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Biot u-p hexa20/8 box {n}^3, heterogeneous log-k, 10% patch load, "
+                        f"SQMR+Jacobi 1e-10, FP32+IR",
+            "n": n, "numel": deck.numel, "nnode": solver.nnode, "neq": solver.neq,
+            "nnz": info["nnz"], "dt": deck.dt, "solver": "sqmr", "precond": "diag",
+            "tol": 1e-10, "setup_s": round(setup_s, 2),
+        },
+        "fp64": {"time_steps_per_s": 1e3 / ms_step64, "ms_per_step": ms_step64,
+                 "krylov_iters_per_step": it64,
+                 "krylov_iters_per_s": it64 / (ms_step64 / 1e3), "max_true_rel": rel64},
+        "fp32ir": {"time_steps_per_s": 1e3 / ms_step32, "ms_per_step": ms_step32,
+                   "inner_iters_per_step": it32,
+                   "krylov_iters_per_s": it32 / (ms_step32 / 1e3), "max_true_rel": rel32,
+                   "max_rel_err_vs_fp64": {"u": err_u, "p": err_p}},
+        "speedup_fp32ir_vs_fp64": ms_step64 / ms_step32,
+        "spmv": {"fp64_ms": ms64, "fp32_ms": ms32, "fp64_GBps": gbps64, "fp32_GBps": gbps32,
+                 "fp64_bytes": b64, "fp32_bytes": b32,
+                 "csr_equivalent_fp64_GBps": csr_bytes(info) / ms64 / 1e6},
+        "roofline": {"bound": "hbm", "kernel": "k_spmv<float> (FP32 node-block SpMV)",
+                     "achieved": gbps32, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": gbps32 / HBM_PEAK_GBPS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

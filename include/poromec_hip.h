@@ -171,6 +171,15 @@ int pm_set_vectors(pm_ctx *ctx, const double *b0, const double *b, const double 
  * porosity_form, update_res_v2. */
 int pm_step(pm_ctx *ctx, double dt, const pm_solver_cfg *cfg, pm_step_info *info);
 
+/* ---- measurement hooks (no counterpart in the reference) ----
+ * Sizes of the device matrix: equations, row blocks (free nodes), stored
+ * values (full matrix) and shared column indices. */
+int pm_matrix_info(pm_ctx *ctx, int *neq, int *nblocks, int64_t *nnz, int64_t *ncols);
+/* Average device time of one SpMV launch (FP64 values, or FP32 values when
+ * use32 != 0) over reps launches, timed with HIP events on the library's
+ * stream. */
+int pm_bench_spmv(pm_ctx *ctx, int use32, int reps, double *ms_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,5 +59,26 @@ def build(verbose: bool = False) -> Path:
     return OUT
 
 
+FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran() -> Path | None:
+    """The Fortran host (iso_c_binding module + time-march driver), linked
+    against libpmhip.so; skipped when amdflang is absent."""
+    fdir = HERE / "fortran"
+    exe = fdir / "pm_mefdrive"
+    if not Path(FC).exists():
+        return None
+    mod_dir = fdir / "_build"
+    mod_dir.mkdir(exist_ok=True)
+    cmd = [FC, "-O2", "-J", str(mod_dir), str(fdir / "pmhip.f90"), str(fdir / "pm_mefdrive.f90"),
+           "-L", str(HERE), "-lpmhip", "-Wl,-rpath,$ORIGIN/..", "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stderr}")
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran())

@@ -695,3 +695,31 @@ extern "C" int pm_step(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_i
   pm_step_info local;
   return pm_step_dev(c, dt, cfg, info ? info : &local);
 }
+
+extern "C" int pm_matrix_info(pm_ctx *c, int *neq, int *nblocks, int64_t *nnz, int64_t *ncols) {
+  if (!c || !c->have_matrix) return PM_ERR_STATE;
+  if (neq) *neq = c->A.neq;
+  if (nblocks) *nblocks = c->A.nrn;
+  if (nnz) *nnz = c->A.nnz;
+  if (ncols) *ncols = c->A.ncols;
+  return PM_OK;
+}
+
+extern "C" int pm_bench_spmv(pm_ctx *c, int use32, int reps, double *ms_per_launch) {
+  if (!c || reps <= 0 || !ms_per_launch) return PM_ERR_ARG;
+  if (!c->have_matrix || !c->A.values_valid) return PM_ERR_STATE;
+  PM_CHECK_HIP(hipSetDevice(c->device));
+  if (use32 && !c->A.have32) PM_TRY(pm_make_val32(c));
+  PM_TRY(pm_ensure_workspace(c, c->A.neq));
+  double *x = c->d_wk[0], *y = c->d_wk[1];
+  PM_CHECK_HIP(hipMemsetAsync(x, 0, sizeof(double) * c->A.neq, c->stream));
+  PM_TRY(pm_spmv_dev(c, x, y, use32));  // warm-up
+  PM_CHECK_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; r++) PM_TRY(pm_spmv_dev(c, (r & 1) ? y : x, (r & 1) ? x : y, use32));
+  PM_CHECK_HIP(hipEventRecord(c->ev1, c->stream));
+  PM_CHECK_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  PM_CHECK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *ms_per_launch = ms / reps;
+  return PM_OK;
+}

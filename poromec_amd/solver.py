@@ -170,6 +170,17 @@ class MefSolver:
                     used=list(info.used[:n]), true_rel=list(info.true_rel[:n]),
                     ms_total=info.ms_total, ms_solve=info.ms_solve, ms_element=info.ms_element)
 
+    def matrix_info(self):
+        neq, nb, nnz, ncols = C.c_int(), C.c_int(), C.c_int64(), C.c_int64()
+        L.check(self.lib.pm_matrix_info(self.ctx, C.byref(neq), C.byref(nb), C.byref(nnz),
+                                        C.byref(ncols)), "pm_matrix_info")
+        return dict(neq=neq.value, nblocks=nb.value, nnz=int(nnz.value), ncols=int(ncols.value))
+
+    def bench_spmv(self, use32: bool, reps: int = 20) -> float:
+        ms = C.c_double()
+        L.check(self.lib.pm_bench_spmv(self.ctx, int(use32), reps, C.byref(ms)), "pm_bench_spmv")
+        return ms.value
+
     def close(self):
         if self.ctx:
             self.lib.pm_ctx_destroy(self.ctx)
