@@ -124,6 +124,9 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--cpu-sample-n", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="set up, then only time the FP64 / FP32 SpMV and a copy kernel "
+                         "(the run profiled with hardware counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this path has no multi-GPU form "
@@ -143,6 +146,19 @@ def main():
     info = solver.matrix_info()
     log(f"setup {setup_s:.1f} s: nnode {solver.nnode}, neq {solver.neq}, nnz {info['nnz']}, "
         f"blocks {info['nblocks']}")
+
+    if args.spmv_only:
+        solver.pform(2, True, False, deck.dt, deck.dt)
+        ms64 = solver.bench_spmv(False, args.spmv_reps)
+        ms32 = solver.bench_spmv(True, args.spmv_reps)
+        copy_bytes = 1 << 30
+        msc = solver.bench_copy(copy_bytes, args.spmv_reps)
+        print(json.dumps({"spmv_fp64_ms": ms64, "spmv_fp32_ms": ms32,
+                          "spmv_fp64_bytes": spmv_bytes(info, 8),
+                          "spmv_fp32_bytes": spmv_bytes(info, 4), "copy_ms": msc,
+                          "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes,
+                          "copy_GBps": 2 * copy_bytes / msc / 1e6, **info}), flush=True)
+        return
 
     cfg64 = SolverConfig(precision="fp64", tol=1e-10)
     cfg32 = SolverConfig(precision="fp32ir", tol=1e-10, ir_inner_tol=1e-6)

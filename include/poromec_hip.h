@@ -179,6 +179,9 @@ int pm_matrix_info(pm_ctx *ctx, int *neq, int *nblocks, int64_t *nnz, int64_t *n
  * use32 != 0) over reps launches, timed with HIP events on the library's
  * stream. */
 int pm_bench_spmv(pm_ctx *ctx, int use32, int reps, double *ms_per_launch);
+/* Average device time of a plain copy kernel over `bytes` bytes (reads
+ * bytes, writes bytes): the reference point for HBM traffic counters. */
+int pm_bench_copy(pm_ctx *ctx, int64_t bytes, int reps, double *ms_per_launch);
 
 #ifdef __cplusplus
 }

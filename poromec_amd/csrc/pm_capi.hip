@@ -33,7 +33,7 @@ extern "C" const char *pm_error_string(int code) {
 
 #define FREE(p)       \
   do {                \
-    if (p) hipFree(p); \
+    if (p) (void)hipFree(p); \
     p = nullptr;      \
   } while (0)
 
@@ -565,10 +565,10 @@ extern "C" int pm_values_upload(pm_ctx *c, const double *ad, const double *al) {
   PM_CHECK_HIP(hipGetLastError());
   PM_TRY(pm_extract_diag(c));
   PM_CHECK_HIP(hipStreamSynchronize(c->stream));
-  hipFree(d_ia);
-  hipFree(d_ja);
-  hipFree(d_ad);
-  hipFree(d_al);
+  (void)hipFree(d_ia);
+  (void)hipFree(d_ja);
+  (void)hipFree(d_ad);
+  (void)hipFree(d_al);
   A.values_valid = true;
   A.have32 = false;
   return PM_OK;
@@ -721,5 +721,35 @@ extern "C" int pm_bench_spmv(pm_ctx *c, int use32, int reps, double *ms_per_laun
   float ms = 0.f;
   PM_CHECK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   *ms_per_launch = ms / reps;
+  return PM_OK;
+}
+
+__global__ __launch_bounds__(256) void k_copy_bench(int64_t n, const double4 *__restrict__ a,
+                                                   double4 *__restrict__ b) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    b[i] = a[i];
+}
+
+extern "C" int pm_bench_copy(pm_ctx *c, int64_t bytes, int reps, double *ms_per_launch) {
+  if (!c || bytes < 32 || reps <= 0 || !ms_per_launch) return PM_ERR_ARG;
+  PM_CHECK_HIP(hipSetDevice(c->device));
+  const int64_t n = bytes / 32;
+  double4 *a = nullptr, *b = nullptr;
+  PM_CHECK_HIP(hipMalloc(&a, n * 32));
+  PM_CHECK_HIP(hipMalloc(&b, n * 32));
+  PM_CHECK_HIP(hipMemsetAsync(a, 0, n * 32, c->stream));
+  const int g = 8 * c->cu_count;
+  hipLaunchKernelGGL(k_copy_bench, dim3(g), dim3(256), 0, c->stream, n, a, b);
+  PM_CHECK_HIP(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_copy_bench, dim3(g), dim3(256), 0, c->stream, n, a, b);
+  PM_CHECK_HIP(hipEventRecord(c->ev1, c->stream));
+  PM_CHECK_HIP(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  PM_CHECK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *ms_per_launch = ms / reps;
+  (void)hipFree(a);
+  (void)hipFree(b);
   return PM_OK;
 }

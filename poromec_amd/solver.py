@@ -181,6 +181,11 @@ class MefSolver:
         L.check(self.lib.pm_bench_spmv(self.ctx, int(use32), reps, C.byref(ms)), "pm_bench_spmv")
         return ms.value
 
+    def bench_copy(self, nbytes: int, reps: int = 20) -> float:
+        ms = C.c_double()
+        L.check(self.lib.pm_bench_copy(self.ctx, int(nbytes), reps, C.byref(ms)), "pm_bench_copy")
+        return ms.value
+
     def close(self):
         if self.ctx:
             self.lib.pm_ctx_destroy(self.ctx)
