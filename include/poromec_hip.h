@@ -171,6 +171,20 @@ int pm_set_vectors(pm_ctx *ctx, const double *b0, const double *b, const double 
  * porosity_form, update_res_v2. */
 int pm_step(pm_ctx *ctx, double dt, const pm_solver_cfg *cfg, pm_step_info *info);
 
+/* ---- batching form (no counterpart in the reference: the reference runs one
+ * problem per process) ----
+ * The mesh given to pm_mesh_setup holds nbox disconnected boxes, box b owning
+ * elements [elem_off[b], elem_off[b+1]) and, after RCM, a contiguous range
+ * of equations.  Afterwards pm_solve and pm_step treat every box as its own
+ * system -- own Krylov scalars, own Krylov and nonlinear stop tests -- while
+ * all boxes share each kernel launch.  Solvers: PCG and SQMR. */
+int pm_batch_define(pm_ctx *ctx, int nbox, const int *elem_off);
+/* Krylov iterations of every box in the last solve (nbox). */
+int pm_batch_iters(pm_ctx *ctx, int *iters);
+/* 1 (default): replay the iteration chunks of batched solves from a captured
+ * hipGraph; 0: launch them one by one. */
+int pm_batch_graphs(pm_ctx *ctx, int on);
+
 /* ---- measurement hooks (no counterpart in the reference) ----
  * Sizes of the device matrix: equations, row blocks (free nodes), stored
  * values (full matrix) and shared column indices. */

@@ -88,6 +88,9 @@ def lib():
     L.pm_get_vectors.argtypes = [_vp, _vp, _vp, _vp]
     L.pm_set_vectors.argtypes = [_vp, _vp, _vp, _vp]
     L.pm_step.argtypes = [_vp, C.c_double, C.POINTER(SolverCfg), C.POINTER(StepInfo)]
+    L.pm_batch_define.argtypes = [_vp, C.c_int, _vp]
+    L.pm_batch_iters.argtypes = [_vp, _vp]
+    L.pm_batch_graphs.argtypes = [_vp, C.c_int]
     L.pm_matrix_info.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int64)]
     L.pm_bench_spmv.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]

@@ -1,0 +1,511 @@
+// pm_batch.hip -- the batching form: many independent Biot boxes held as one
+// block-diagonal matrix and solved together.
+//
+// The boxes arrive as one disconnected mesh.  RCM numbers each connected
+// component in turn (Reord.f genrcm_v1), so every box owns a contiguous
+// range of equations and of node blocks.  Each launch covers all boxes:
+// workgroups are assigned to boxes in proportion to their rows, every box
+// has its own Krylov scalars, reductions and stop test (the reference's
+// recurrences of pcg / rpsqmr, Iters.f:299-595, 6100-6436), and workgroups of
+// a converged box return at entry, so finished boxes cost nothing.  Chunks of
+// iterations are captured once in a hipGraph and replayed.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "pm_functors.h"
+
+struct PmBatch {
+  int nbox = 0;
+  std::vector<int> eq0, blk0, wg0;  // nbox + 1
+  int G = 0;
+  int *d_wg_box = nullptr, *d_wg0 = nullptr, *d_eq0 = nullptr, *d_blk0 = nullptr;
+  unsigned int *d_counters = nullptr;
+  double *d_partials = nullptr;
+  PmScalars *d_S = nullptr, *h_S = nullptr;
+  int *d_ndone = nullptr, *h_ndone = nullptr;
+  std::vector<int> last_iters;
+  // hipGraph cache of one chunk of iterations
+  hipGraphExec_t graph = nullptr;
+  uintptr_t graph_key[6] = {0, 0, 0, 0, 0, 0};
+  int use_graph = 1;
+  ~PmBatch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    (void)hipFree(d_wg_box);
+    (void)hipFree(d_wg0);
+    (void)hipFree(d_eq0);
+    (void)hipFree(d_blk0);
+    (void)hipFree(d_counters);
+    (void)hipFree(d_partials);
+    (void)hipFree(d_S);
+    (void)hipFree(d_ndone);
+    if (h_S) (void)hipHostFree(h_S);
+    if (h_ndone) (void)hipHostFree(h_ndone);
+  }
+};
+
+void pm_batch_free(PmBatch *b) { delete b; }
+
+struct BatchArgs {
+  const int *wg_box, *wg0, *eq0, *blk0;
+  unsigned int *counters;
+  double *partials;
+  int *ndone;
+};
+
+// reduction over the workgroups [gbase, gbase + gcount) of one box
+template <int NV>
+__device__ __forceinline__ bool pm_group_reduce(const double *v, double *partials,
+                                                unsigned int *counter, int gbase, int gcount,
+                                                double *out) {
+  constexpr int NS = NV > 0 ? NV : 1;
+  __shared__ double sh[NS][PM_WAVES];
+  __shared__ int amlast;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double w = pm_wave_sum(v[i]);
+    if (lane == 0) sh[i][wave] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      double acc = sh[i][0];
+      for (int w = 1; w < PM_WAVES; w++) acc += sh[i][w];
+      partials[(size_t)blockIdx.x * NS + i] = acc;
+    }
+    __threadfence();
+    unsigned int t = atomicAdd(counter, 1u);
+    amlast = (t == (unsigned)gcount - 1u);
+  }
+  __syncthreads();
+  if (!amlast) return false;
+  __threadfence();
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < gcount; b += PM_WG)
+      acc += __builtin_nontemporal_load(&partials[(size_t)(gbase + b) * NS + i]);
+    acc = pm_wave_sum(acc);
+    __syncthreads();
+    if (lane == 0) sh[i][wave] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double acc = sh[i][0];
+    for (int w = 1; w < PM_WAVES; w++) acc += sh[i][w];
+    out[i] = acc;
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+template <class Epi>
+struct CountDone {  // epilogue wrapper: counts boxes that finish
+  Epi e;
+  int *ndone;
+  __device__ void operator()(PmScalars *S, const double *r) const {
+    const int was = S->done;
+    e(S, r);
+    if (!was && S->done) atomicAdd(ndone, 1);
+  }
+};
+
+template <typename VT, int NV, class Epi>
+__global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, const int *__restrict__ fr,
+                                                  const int *__restrict__ nr,
+                                                  const int64_t *__restrict__ colptr,
+                                                  const int64_t *__restrict__ valptr,
+                                                  const int *__restrict__ ncol,
+                                                  const VT *__restrict__ val,
+                                                  const double *__restrict__ x,
+                                                  double *__restrict__ y,
+                                                  const double *__restrict__ w, PmScalars *S,
+                                                  Epi epi) {
+  const int box = B.wg_box[blockIdx.x];
+  PmScalars *Sb = S + box;
+  if (Sb->done) return;
+  const int gb = B.wg0[box], gc = B.wg0[box + 1] - gb;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t nb0 = B.blk0[box], nblk = B.blk0[box + 1] - nb0;
+  int64_t k0, k1;
+  pm_chunk(nblk, blockIdx.x - gb, gc, k0, k1);
+  double d0 = 0.0, d1 = 0.0;
+  for (int64_t kk = k0 + wave; kk < k1; kk += PM_WAVES) {
+    const int64_t k = nb0 + kk;
+    const int64_t cp = colptr[k];
+    const int c = (int)(colptr[k + 1] - cp);
+    const int64_t vp = valptr[k];
+    const int n = nr[k];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int j = lane; j < c; j += 64) {
+      const int col = __builtin_nontemporal_load(ncol + cp + j);
+      const double xv = x[col];
+      const VT *v = val + vp + j;
+      a0 += (double)__builtin_nontemporal_load(v) * xv;
+      if (n > 1) a1 += (double)__builtin_nontemporal_load(v + c) * xv;
+      if (n > 2) a2 += (double)__builtin_nontemporal_load(v + 2 * c) * xv;
+      if (n > 3) a3 += (double)__builtin_nontemporal_load(v + 3 * c) * xv;
+    }
+    a0 = pm_wave_sum(a0);
+    if (n > 1) a1 = pm_wave_sum(a1);
+    if (n > 2) a2 = pm_wave_sum(a2);
+    if (n > 3) a3 = pm_wave_sum(a3);
+    if (lane == 0) {
+      const int f = fr[k];
+      const double r[4] = {a0, a1, a2, a3};
+      for (int q = 0; q < n; q++) {
+        y[f + q] = r[q];
+        if (NV >= 1) d0 += r[q] * w[f + q];
+        if (NV >= 2) d1 += r[q] * r[q];
+      }
+    }
+  }
+  if (NV > 0) {
+    double v[2] = {d0, d1}, out[2];
+    if (pm_group_reduce<NV>(v, B.partials, B.counters + box, gb, gc, out) && threadIdx.x == 0) {
+      for (int i = 0; i < NV; i++) Sb->red[i] = out[i];
+      epi(Sb, out);
+    }
+  }
+}
+
+template <int NV, class Body, class Epi>
+__global__ __launch_bounds__(PM_WG) void k_vec_b(BatchArgs B, PmScalars *S, Body body, Epi epi,
+                                                 int ignore_done) {
+  const int box = B.wg_box[blockIdx.x];
+  PmScalars *Sb = S + box;
+  if (!ignore_done && Sb->done) return;
+  const int gb = B.wg0[box], gc = B.wg0[box + 1] - gb;
+  const int64_t r0 = B.eq0[box], nrow = B.eq0[box + 1] - r0;
+  int64_t a, e;
+  pm_chunk(nrow, blockIdx.x - gb, gc, a, e);
+  double acc[NV > 0 ? NV : 1];
+  for (int i = 0; i < (NV > 0 ? NV : 1); i++) acc[i] = 0.0;
+  const PmScalars &s = *Sb;
+  for (int64_t i = a + threadIdx.x; i < e; i += PM_WG) body(r0 + i, s, acc);
+  double out[NV > 0 ? NV : 1];
+  if (pm_group_reduce<NV>(acc, B.partials, B.counters + box, gb, gc, out) && threadIdx.x == 0) {
+    for (int i = 0; i < NV; i++) Sb->red[i] = out[i];
+    epi(Sb, out);
+  }
+}
+
+// a box whose right-hand side is zero has x = 0 and is done at once
+template <class Epi>
+struct ZeroRhsDone {
+  Epi e;
+  int *ndone;
+  __device__ void operator()(PmScalars *S, const double *r) const {
+    e(S, r);
+    if (r[0] == 0.0) {
+      S->done = 1;
+      S->iter = 0;
+      atomicAdd(ndone, 1);
+    }
+  }
+};
+
+static BatchArgs batch_args(PmBatch *B) {
+  return BatchArgs{B->d_wg_box, B->d_wg0, B->d_eq0, B->d_blk0, B->d_counters, B->d_partials,
+                   B->d_ndone};
+}
+
+template <int NV, class Epi>
+static void bspmv(pm_ctx *c, int use32, const double *x, double *y, const double *w, Epi epi) {
+  PmBatch *B = c->batch;
+  const PmDevMatrix &A = c->A;
+  if (use32)
+    hipLaunchKernelGGL((k_spmv_b<float, NV, Epi>), dim3(B->G), dim3(PM_WG), 0, c->stream,
+                       batch_args(B), A.fr, A.nr, A.colptr, A.valptr, A.ncol, A.val32, x, y, w,
+                       B->d_S, epi);
+  else
+    hipLaunchKernelGGL((k_spmv_b<double, NV, Epi>), dim3(B->G), dim3(PM_WG), 0, c->stream,
+                       batch_args(B), A.fr, A.nr, A.colptr, A.valptr, A.ncol, A.val, x, y, w,
+                       B->d_S, epi);
+}
+
+template <int NV, class Body, class Epi>
+static void bvec(pm_ctx *c, Body body, Epi epi, int ignore_done = 0) {
+  PmBatch *B = c->batch;
+  hipLaunchKernelGGL((k_vec_b<NV, Body, Epi>), dim3(B->G), dim3(PM_WG), 0, c->stream,
+                     batch_args(B), B->d_S, body, epi, ignore_done);
+}
+
+static int breset(pm_ctx *c) {
+  PmBatch *B = c->batch;
+  PM_CHECK_HIP(hipMemsetAsync(B->d_S, 0, sizeof(PmScalars) * B->nbox, c->stream));
+  PM_CHECK_HIP(hipMemsetAsync(B->d_ndone, 0, sizeof(int), c->stream));
+  return PM_OK;
+}
+
+static int bpoll(pm_ctx *c, int *ndone) {
+  PmBatch *B = c->batch;
+  PM_CHECK_HIP(hipMemcpyAsync(B->h_ndone, B->d_ndone, sizeof(int), hipMemcpyDeviceToHost,
+                              c->stream));
+  PM_CHECK_HIP(hipStreamSynchronize(c->stream));
+  *ndone = *B->h_ndone;
+  return PM_OK;
+}
+
+// one chunk of iterations of the batched solver
+static void bchunk(pm_ctx *c, int solver, int use32, int chunk, const double *m, double *x) {
+  PmBatch *B = c->batch;
+  double **w = c->d_wk;
+  for (int it = 0; it < chunk; it++) {
+    if (solver == PM_SOLVER_PCG) {
+      double *r = w[0], *z = w[1], *p = w[2];
+      bspmv<1>(c, use32, p, z, p, PcgAlpha{});
+      bvec<1>(c, PcgUpd{x, r, z, p, m}, PcgUpdEpi{});
+      bvec<0>(c, PcgP{p, z}, CountDone<FinishEpi>{FinishEpi{}, B->d_ndone});
+    } else {
+      double *t = w[0], *r = w[1], *q = w[2], *d = w[3];
+      bspmv<1>(c, use32, q, t, q, CountDone<SqmrSigma>{SqmrSigma{}, B->d_ndone});
+      bvec<1>(c, SqmrR{r, t}, SqmrREpi{});
+      bvec<1>(c, SqmrD{d, x, t, q, r, m}, CountDone<SqmrDEpi>{SqmrDEpi{}, B->d_ndone});
+      bvec<0>(c, SqmrQ{q, t}, NoEpi{});
+    }
+  }
+}
+
+// batched solve of every box: b, x, m device vectors of neq
+int pm_batch_krylov(pm_ctx *c, int solver, int maxit, double tol, const double *b, double *x,
+                    const double *m, int use32, pm_solve_info *info) {
+  PmBatch *B = c->batch;
+  if (solver != PM_SOLVER_SQMR && solver != PM_SOLVER_PCG) return PM_ERR_UNSUPPORTED;
+  PM_TRY(breset(c));
+  if (solver == PM_SOLVER_PCG) {
+    double *r = c->d_wk[0], *z = c->d_wk[1], *p = c->d_wk[2];
+    bvec<1>(c, PcgInit{b, m, x, r, z, p}, ZeroRhsDone<PcgInitEpi>{PcgInitEpi{tol}, B->d_ndone}, 1);
+  } else {
+    double *r = c->d_wk[1], *q = c->d_wk[2], *d = c->d_wk[3];
+    bvec<2>(c, SqmrInit{b, m, x, r, q, d}, ZeroRhsDone<SqmrInitEpi>{SqmrInitEpi{tol}, B->d_ndone},
+            1);
+  }
+  int launched = 0, ndone = 0;
+  PM_TRY(bpoll(c, &ndone));
+  const int chunk = 16;
+  // graph of one chunk: kernels only, fixed pointers for this (solver, precision, x, m)
+  const uintptr_t key[6] = {(uintptr_t)solver, (uintptr_t)use32, (uintptr_t)x, (uintptr_t)m,
+                            (uintptr_t)c->d_wk[0], (uintptr_t)c->A.val32};
+  if (B->use_graph && (B->graph == nullptr || memcmp(key, B->graph_key, sizeof(key)) != 0)) {
+    if (B->graph) (void)hipGraphExecDestroy(B->graph);
+    B->graph = nullptr;
+    hipGraph_t g;
+    PM_CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    bchunk(c, solver, use32, chunk, m, x);
+    PM_CHECK_HIP(hipStreamEndCapture(c->stream, &g));
+    PM_CHECK_HIP(hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    memcpy(B->graph_key, key, sizeof(key));
+  }
+  while (ndone < B->nbox && launched < maxit) {
+    if (B->use_graph) PM_CHECK_HIP(hipGraphLaunch(B->graph, c->stream));
+    else bchunk(c, solver, use32, chunk, m, x);
+    launched += chunk;
+    (use32 ? info->spmv32 : info->spmv64) += chunk;
+    PM_CHECK_HIP(hipGetLastError());
+    PM_TRY(bpoll(c, &ndone));
+  }
+  PM_CHECK_HIP(hipMemcpyAsync(B->h_S, B->d_S, sizeof(PmScalars) * B->nbox, hipMemcpyDeviceToHost,
+                              c->stream));
+  PM_CHECK_HIP(hipStreamSynchronize(c->stream));
+  B->last_iters.assign(B->nbox, 0);
+  int worst = 0, maxiter = 0;
+  double maxres = 0.0;
+  for (int k = 0; k < B->nbox; k++) {
+    const PmScalars &s = B->h_S[k];
+    if (s.done < 0) worst = s.done;
+    else if (s.done == 0 && worst == 0) worst = PM_ERR_MAXIT;
+    B->last_iters[k] = s.iter;
+    maxiter = std::max(maxiter, s.iter);
+    maxres = std::max(maxres, s.resid);
+  }
+  info->iters += maxiter;
+  info->resid = maxres;
+  info->used_solver = solver;
+  return worst;
+}
+
+// per-box ||R_u||^2, ||R_p||^2 (red[0], red[1] of every box's scalars)
+struct ResNormBody {
+  const double *b;
+  const unsigned char *isp;
+  __device__ void operator()(int64_t i, const PmScalars &, double *acc) const {
+    const double bi = b[i];
+    if (isp[i]) acc[1] += bi * bi;
+    else acc[0] += bi * bi;
+  }
+};
+struct ZeroBoxBody {  // clears the right-hand side of converged boxes
+  double *b;
+  const int *active;
+  const int *eq_box;
+  __device__ void operator()(int64_t i, const PmScalars &, double *) const {
+    if (!active[eq_box[i]]) b[i] = 0.0;
+  }
+};
+
+int pm_batch_resnorms(pm_ctx *c, const double *b, std::vector<double> &ru2,
+                      std::vector<double> &rp2) {
+  PmBatch *B = c->batch;
+  PM_TRY(breset(c));
+  bvec<2>(c, ResNormBody{b, c->d_isp}, NoEpi{}, 1);
+  PM_CHECK_HIP(hipMemcpyAsync(B->h_S, B->d_S, sizeof(PmScalars) * B->nbox, hipMemcpyDeviceToHost,
+                              c->stream));
+  PM_CHECK_HIP(hipStreamSynchronize(c->stream));
+  ru2.resize(B->nbox);
+  rp2.resize(B->nbox);
+  for (int k = 0; k < B->nbox; k++) {
+    ru2[k] = B->h_S[k].red[0];
+    rp2[k] = B->h_S[k].red[1];
+  }
+  return PM_OK;
+}
+
+int pm_batch_mask_rhs(pm_ctx *c, double *b, const std::vector<int> &active) {
+  PmBatch *B = c->batch;
+  int *d_active = nullptr;
+  PM_CHECK_HIP(hipMalloc(&d_active, sizeof(int) * B->nbox));
+  PM_CHECK_HIP(hipMemcpyAsync(d_active, active.data(), sizeof(int) * B->nbox,
+                              hipMemcpyHostToDevice, c->stream));
+  bvec<0>(c, ZeroBoxBody{b, d_active, c->d_eq_box}, NoEpi{}, 1);
+  PM_CHECK_HIP(hipStreamSynchronize(c->stream));
+  (void)hipFree(d_active);
+  return PM_OK;
+}
+
+int pm_batch_define_dev(pm_ctx *c, int nbox, const int *elem_off) {
+  if (!c->have_mesh) return PM_ERR_STATE;
+  const PmHostMesh &m = c->mesh;
+  if (nbox < 1 || elem_off[0] != 0 || elem_off[nbox] != m.numel) return PM_ERR_ARG;
+  // equation range of every box from its elements' equations
+  std::vector<int> lo(nbox, INT32_MAX), hi(nbox, -1);
+  for (int b = 0; b < nbox; b++) {
+    if (elem_off[b + 1] <= elem_off[b]) return PM_ERR_ARG;
+    for (int e = elem_off[b]; e < elem_off[b + 1]; e++)
+      for (int k = 0; k < 20; k++) {
+        const int n = m.ix20[(size_t)e * 21 + k] - 1;
+        const int nd = m.fnno[n] == 1 ? 4 : 3;
+        for (int d = 0; d < nd; d++) {
+          const int q = m.id[(size_t)n * 4 + d];
+          if (q > 0) {
+            lo[b] = std::min(lo[b], q - 1);
+            hi[b] = std::max(hi[b], q - 1);
+          }
+        }
+      }
+  }
+  PmBatch *B = new PmBatch();
+  B->nbox = nbox;
+  B->eq0.assign(nbox + 1, 0);
+  for (int b = 0; b < nbox; b++) {
+    if (hi[b] < 0 || lo[b] != B->eq0[b]) {  // boxes must hold consecutive equations
+      delete B;
+      return PM_ERR_ARG;
+    }
+    B->eq0[b + 1] = hi[b] + 1;
+  }
+  if (B->eq0[nbox] != m.neq) {
+    delete B;
+    return PM_ERR_ARG;
+  }
+  // node blocks of every box (blocks are in equation order)
+  std::vector<int> eqblk(m.neq);
+  PM_CHECK_HIP(hipMemcpy(eqblk.data(), c->A.eq_block, sizeof(int) * m.neq, hipMemcpyDeviceToHost));
+  B->blk0.assign(nbox + 1, 0);
+  for (int b = 0; b < nbox; b++) B->blk0[b] = eqblk[B->eq0[b]];
+  B->blk0[nbox] = c->A.nrn;
+  // workgroups in proportion to rows, at least one per box
+  const int target = std::max(2048, nbox);
+  B->wg0.assign(nbox + 1, 0);
+  for (int b = 0; b < nbox; b++) {
+    const double share = (double)(B->eq0[b + 1] - B->eq0[b]) / m.neq;
+    B->wg0[b + 1] = B->wg0[b] + std::max(1, (int)(share * target));
+  }
+  B->G = B->wg0[nbox];
+  std::vector<int> wg_box(B->G);
+  for (int b = 0; b < nbox; b++)
+    for (int g = B->wg0[b]; g < B->wg0[b + 1]; g++) wg_box[g] = b;
+  std::vector<int> eq_box(m.neq);
+  for (int b = 0; b < nbox; b++)
+    for (int q = B->eq0[b]; q < B->eq0[b + 1]; q++) eq_box[q] = b;
+  auto up = [&](int **d, const std::vector<int> &h) -> int {
+    PM_CHECK_HIP(hipMalloc(d, sizeof(int) * h.size()));
+    PM_CHECK_HIP(hipMemcpy(*d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+    return PM_OK;
+  };
+  int rc = PM_OK;
+  if ((rc = up(&B->d_wg_box, wg_box)) || (rc = up(&B->d_wg0, B->wg0)) ||
+      (rc = up(&B->d_eq0, B->eq0)) || (rc = up(&B->d_blk0, B->blk0))) {
+    delete B;
+    return rc;
+  }
+  if (c->d_eq_box) (void)hipFree(c->d_eq_box);
+  c->d_eq_box = nullptr;
+  if ((rc = up(&c->d_eq_box, eq_box))) {
+    delete B;
+    return rc;
+  }
+  auto chk = [&](hipError_t e) {
+    if (e != hipSuccess && rc == PM_OK) rc = PM_ERR_HIP;
+  };
+  chk(hipMalloc(&B->d_counters, sizeof(unsigned int) * nbox));
+  chk(hipMemset(B->d_counters, 0, sizeof(unsigned int) * nbox));
+  chk(hipMalloc(&B->d_partials, sizeof(double) * 2 * B->G));
+  chk(hipMalloc(&B->d_S, sizeof(PmScalars) * nbox));
+  chk(hipHostMalloc((void **)&B->h_S, sizeof(PmScalars) * nbox, hipHostMallocDefault));
+  chk(hipMalloc(&B->d_ndone, sizeof(int)));
+  chk(hipHostMalloc((void **)&B->h_ndone, sizeof(int), hipHostMallocDefault));
+  if (rc) {
+    delete B;
+    return rc;
+  }
+  if (c->batch) pm_batch_free(c->batch);
+  c->batch = B;
+  return PM_OK;
+}
+
+int pm_batch_nbox(pm_ctx *c) { return c->batch ? c->batch->nbox : 0; }
+
+int pm_batch_last_iters(pm_ctx *c, int *iters) {
+  if (!c->batch) return PM_ERR_STATE;
+  for (int b = 0; b < c->batch->nbox; b++)
+    iters[b] = b < (int)c->batch->last_iters.size() ? c->batch->last_iters[b] : 0;
+  return PM_OK;
+}
+
+void pm_batch_set_graph(pm_ctx *c, int on) {
+  if (c->batch) c->batch->use_graph = on;
+}
+
+// res = b - A x and per-box ||res||^2, ||b||^2
+int pm_batch_true_res(pm_ctx *c, const double *b, const double *x, double *res,
+                      std::vector<double> &r2, std::vector<double> &b2) {
+  PmBatch *B = c->batch;
+  PM_TRY(pm_spmv_dev(c, x, res, 0));
+  PM_TRY(breset(c));
+  struct TrueResB {
+    const double *b;
+    double *r;
+    __device__ void operator()(int64_t i, const PmScalars &, double *acc) const {
+      const double bi = b[i], ri = bi - r[i];
+      r[i] = ri;
+      acc[0] += ri * ri;
+      acc[1] += bi * bi;
+    }
+  };
+  bvec<2>(c, TrueResB{b, res}, NoEpi{}, 1);
+  PM_CHECK_HIP(hipMemcpyAsync(B->h_S, B->d_S, sizeof(PmScalars) * B->nbox, hipMemcpyDeviceToHost,
+                              c->stream));
+  PM_CHECK_HIP(hipStreamSynchronize(c->stream));
+  r2.resize(B->nbox);
+  b2.resize(B->nbox);
+  for (int k = 0; k < B->nbox; k++) {
+    r2[k] = B->h_S[k].red[0];
+    b2[k] = B->h_S[k].red[1];
+  }
+  return PM_OK;
+}

@@ -52,6 +52,8 @@ static void free_matrix(PmDevMatrix &A) {
 }
 
 pm_ctx::~pm_ctx() {
+  if (batch) pm_batch_free(batch);
+  FREE(d_eq_box);
   free_matrix(A);
   FREE(d_ix);
   FREE(d_xv);
@@ -751,5 +753,22 @@ extern "C" int pm_bench_copy(pm_ctx *c, int64_t bytes, int reps, double *ms_per_
   *ms_per_launch = ms / reps;
   (void)hipFree(a);
   (void)hipFree(b);
+  return PM_OK;
+}
+
+extern "C" int pm_batch_define(pm_ctx *c, int nbox, const int *elem_off) {
+  if (!c || !elem_off) return PM_ERR_ARG;
+  PM_CHECK_HIP(hipSetDevice(c->device));
+  return pm_batch_define_dev(c, nbox, elem_off);
+}
+
+extern "C" int pm_batch_iters(pm_ctx *c, int *iters) {
+  if (!c || !iters) return PM_ERR_ARG;
+  return pm_batch_last_iters(c, iters);
+}
+
+extern "C" int pm_batch_graphs(pm_ctx *c, int on) {
+  if (!c || !c->batch) return PM_ERR_STATE;
+  pm_batch_set_graph(c, on);
   return PM_OK;
 }

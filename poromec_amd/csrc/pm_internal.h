@@ -82,7 +82,9 @@ int64_t pm_host_count_lower(const PmHostPattern &p);
 // ---------------------------------------------------------------------------
 // device context
 // ---------------------------------------------------------------------------
-struct PmScalars;  // device-side Krylov scalars (pm_krylov.hip)
+struct PmScalars;  // device-side Krylov scalars (pm_device.h)
+struct PmBatch;    // batching form (pm_batch.hip)
+void pm_batch_free(PmBatch *b);
 
 struct PmDevMatrix {
   int nrn = 0, neq = 0;
@@ -155,6 +157,10 @@ struct pm_ctx {
   PmScalars *h_scal = nullptr; // pinned mirror
   int red_blocks = 1024;
 
+  // batching form
+  PmBatch *batch = nullptr;
+  int *d_eq_box = nullptr;  // (neq) box of every equation
+
   ~pm_ctx();
 };
 
@@ -172,3 +178,14 @@ int pm_extract_diag(pm_ctx *c);
 int pm_make_val32(pm_ctx *c);
 int pm_step_dev(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_info *info);
 double pm_dot_host(pm_ctx *c, const double *a, const double *b, int n, int *rc);
+int pm_batch_define_dev(pm_ctx *c, int nbox, const int *elem_off);
+int pm_batch_nbox(pm_ctx *c);
+int pm_batch_last_iters(pm_ctx *c, int *iters);
+void pm_batch_set_graph(pm_ctx *c, int on);
+int pm_batch_krylov(pm_ctx *c, int solver, int maxit, double tol, const double *b, double *x,
+                    const double *m, int use32, pm_solve_info *info);
+int pm_batch_resnorms(pm_ctx *c, const double *b, std::vector<double> &ru2,
+                      std::vector<double> &rp2);
+int pm_batch_mask_rhs(pm_ctx *c, double *b, const std::vector<int> &active);
+int pm_batch_true_res(pm_ctx *c, const double *b, const double *x, double *res,
+                      std::vector<double> &r2, std::vector<double> &b2);

@@ -15,6 +15,8 @@
 #include <cstring>
 
 #include "pm_device.h"
+#include "pm_functors.h"
+#include <vector>
 
 #define PM_SPMV_MAX_GRID 8192
 
@@ -22,13 +24,6 @@
 // SpMV over node blocks: one wavefront per block, lanes stride the shared
 // column list, every lane carries the (<= 4) rows of the block.
 // ---------------------------------------------------------------------------
-struct NoEpi {
-  __device__ void operator()(PmScalars *, const double *) const {}
-};
-struct NoBody {
-  __device__ void operator()(int64_t, const PmScalars &, double *) const {}
-};
-
 template <typename VT, int NV, bool SCALE_M, class Epi>
 __global__ __launch_bounds__(PM_WG) void k_spmv(
     int nrn, const int *__restrict__ fr, const int *__restrict__ nr,
@@ -141,12 +136,6 @@ static void launch_vec(pm_ctx *c, int64_t n, PmScalars *S, Body body, Epi epi,
                      c->d_partials, c->d_counter, body, epi, ignore_done);
 }
 
-// scalar slot names
-enum {
-  S_ALPHA = 0, S_BETA, S_RO, S_TAU, S_V0, S_T1, S_T2, S_T3, S_T4, S_D, S_NORMR, S_NETA,
-  S_BETAOLD, S_COLD, S_C, S_SOLD, S_S, S_W, S_RR0, S_NB, S_E0
-};
-
 static int poll_done(pm_ctx *c, int *done, int *iter) {
   PM_CHECK_HIP(hipMemcpyAsync(c->h_scal, c->d_scal, sizeof(PmScalars), hipMemcpyDeviceToHost,
                               c->stream));
@@ -171,166 +160,6 @@ static int next_chunk(int launched) {
   if (launched < 256) return 32;
   return 64;
 }
-
-// ---------------------------------------------------------------------------
-// PCG (Iters.f:299-595)
-// ---------------------------------------------------------------------------
-struct PcgInit {
-  const double *b, *m;
-  double *x, *r, *z, *p;
-  __device__ void operator()(int64_t i, const PmScalars &, double *acc) const {
-    double bi = b[i], zi = bi * m[i];
-    x[i] = 0.0;
-    r[i] = bi;
-    z[i] = zi;
-    p[i] = zi;
-    acc[0] += bi * zi;
-  }
-};
-struct PcgInitEpi {
-  double tol;
-  __device__ void operator()(PmScalars *S, const double *r) const {
-    S->conv = tol * sqrt(fabs(r[0]));
-    S->s[S_D] = r[0];
-    S->it = 0;
-  }
-};
-struct PcgAlpha {
-  __device__ void operator()(PmScalars *S, const double *r) const {
-    S->it += 1;
-    S->s[S_ALPHA] = S->s[S_D] / r[0];
-  }
-};
-struct PcgUpd {
-  double *x, *r, *z;
-  const double *p, *m;
-  __device__ void operator()(int64_t i, const PmScalars &s, double *acc) const {
-    double al = s.s[S_ALPHA];
-    x[i] = x[i] + al * p[i];
-    double ri = r[i] - al * z[i];
-    r[i] = ri;
-    double zi = ri * m[i];
-    z[i] = zi;
-    acc[0] += ri * zi;
-  }
-};
-struct PcgUpdEpi {
-  __device__ void operator()(PmScalars *S, const double *r) const {
-    double di = r[0];
-    S->s[S_BETA] = di / S->s[S_D];
-    S->s[S_D] = di;
-    S->resid = sqrt(fabs(di));
-    if (sqrt(fabs(di)) < S->conv) S->pending = 1;
-  }
-};
-struct PcgP {
-  double *p;
-  const double *z;
-  __device__ void operator()(int64_t i, const PmScalars &s, double *) const {
-    p[i] = z[i] + s.s[S_BETA] * p[i];
-  }
-};
-struct FinishEpi {  // turns a pending convergence into done after the update
-  __device__ void operator()(PmScalars *S, const double *) const {
-    if (S->pending) {
-      S->done = 1;
-      S->iter = S->it;
-    }
-  }
-};
-
-// ---------------------------------------------------------------------------
-// RPSQMR (Iters.f:6100-6436)
-// ---------------------------------------------------------------------------
-struct SqmrInit {
-  const double *b, *m;
-  double *x, *r, *q, *d;
-  __device__ void operator()(int64_t i, const PmScalars &, double *acc) const {
-    double bi = b[i];
-    x[i] = 0.0;
-    r[i] = bi;
-    double qi = bi * m[i];
-    q[i] = qi;
-    d[i] = 0.0;
-    acc[0] += bi * bi;
-    acc[1] += bi * qi;
-  }
-};
-struct SqmrInitEpi {
-  double tol;
-  __device__ void operator()(PmScalars *S, const double *r) const {
-    S->conv = tol * sqrt(fabs(r[0]));
-    S->s[S_TAU] = sqrt(r[0]);
-    S->s[S_RO] = r[1];
-    S->s[S_V0] = 0.0;
-    S->it = 0;
-  }
-};
-struct SqmrSigma {
-  __device__ void operator()(PmScalars *S, const double *r) const {
-    S->it += 1;
-    double sigma = r[0];
-    if (sigma == 0.0) {
-      S->done = PM_ERR_BREAKDOWN;
-      return;
-    }
-    S->s[S_ALPHA] = S->s[S_RO] / sigma;
-  }
-};
-struct SqmrR {
-  double *r;
-  const double *t;
-  __device__ void operator()(int64_t i, const PmScalars &s, double *acc) const {
-    double ri = r[i] - s.s[S_ALPHA] * t[i];
-    r[i] = ri;
-    acc[0] += ri * ri;
-  }
-};
-struct SqmrREpi {
-  __device__ void operator()(PmScalars *S, const double *r) const {
-    double vn = sqrt(r[0]) / S->s[S_TAU];
-    double cn = 1.0 / sqrt(1.0 + vn * vn);
-    S->s[S_TAU] = S->s[S_TAU] * vn * cn;
-    S->s[S_T1] = cn * cn * S->s[S_V0] * S->s[S_V0];
-    S->s[S_T2] = cn * cn * S->s[S_ALPHA];
-    S->s[S_V0] = vn;
-    S->s[S_NORMR] = sqrt(r[0]);
-  }
-};
-struct SqmrD {
-  double *d, *x, *t;
-  const double *q, *r, *m;
-  __device__ void operator()(int64_t i, const PmScalars &s, double *acc) const {
-    double di = s.s[S_T1] * d[i] + s.s[S_T2] * q[i];
-    d[i] = di;
-    x[i] = x[i] + di;
-    double ri = r[i], ti = ri * m[i];
-    t[i] = ti;
-    acc[0] += ri * ti;
-  }
-};
-struct SqmrDEpi {
-  __device__ void operator()(PmScalars *S, const double *r) const {
-    if (S->s[S_RO] == 0.0) {
-      S->done = PM_ERR_BREAKDOWN;
-      return;
-    }
-    S->s[S_BETA] = r[0] / S->s[S_RO];
-    S->s[S_RO] = r[0];
-    S->resid = S->s[S_NORMR];
-    if (S->s[S_NORMR] < S->conv) {
-      S->done = 1;
-      S->iter = S->it;
-    }
-  }
-};
-struct SqmrQ {
-  double *q;
-  const double *t;
-  __device__ void operator()(int64_t i, const PmScalars &s, double *) const {
-    q[i] = t[i] + s.s[S_BETA] * q[i];
-  }
-};
 
 // ---------------------------------------------------------------------------
 // PBiCGSTAB (Iters.f:2817-3105), right Jacobi preconditioning
@@ -1056,6 +885,61 @@ int pm_solve_dev(pm_ctx *c, const pm_solver_cfg *cfg, const double *b, double *x
   PM_TRY(read_red(c, &nzero, nullptr));
   if (nzero > 0.0) return PM_ERR_ZERO_DIAG;
   double *res = c->d_wk[12];
+  if (c->batch) {  // every box its own system, all boxes in the same launches
+    if (cfg->precision == PM_PREC_FP64) {
+      PM_TRY(pm_batch_krylov(c, cfg->solver, cfg->maxit, cfg->tol, b, x, c->d_m, 0, info));
+    } else {
+      if (!c->A.have32) PM_TRY(pm_make_val32(c));
+      double *rr = c->d_wk[13], *dx = c->d_wk[14], *m32 = c->d_wk[15];
+      launch_vec<0>(c, n, c->d_scal, RoundM32{c->d_m, m32}, NoEpi{}, 1);
+      launch_vec<0>(c, n, c->d_scal, CopyBody{rr, b}, NoEpi{}, 1);
+      PM_CHECK_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, c->stream));
+      const int nbox = pm_batch_nbox(c);
+      std::vector<double> r2, b2;
+      std::vector<int> active(nbox, 1);
+      const int maxouter = cfg->ir_maxouter > 0 ? cfg->ir_maxouter : 10;
+      const double inner = cfg->ir_inner_tol > 0 ? cfg->ir_inner_tol : 1e-6;
+      bool all = false;
+      for (int o = 0; o < maxouter && !all; o++) {
+        pm_solve_info sub;
+        memset(&sub, 0, sizeof(sub));
+        PM_TRY(pm_batch_krylov(c, cfg->solver, cfg->maxit, inner, rr, dx, m32, 1, &sub));
+        info->iters += sub.iters;
+        info->spmv32 += sub.spmv32;
+        info->outer++;
+        launch_vec<0>(c, n, c->d_scal, AxpyBody{x, dx}, NoEpi{}, 1);
+        PM_TRY(pm_batch_true_res(c, b, x, res, r2, b2));
+        info->spmv64++;
+        all = true;
+        double worst = 0.0;
+        for (int k = 0; k < nbox; k++) {
+          const double rel = b2[k] > 0 ? sqrt(r2[k] / b2[k]) : 0.0;
+          active[k] = rel >= cfg->tol;
+          all = all && !active[k];
+          worst = std::max(worst, rel);
+        }
+        info->resid = worst;
+        if (!all) {
+          launch_vec<0>(c, n, c->d_scal, CopyBody{rr, res}, NoEpi{}, 1);
+          PM_TRY(pm_batch_mask_rhs(c, rr, active));
+        }
+      }
+      if (!all) return PM_ERR_MAXIT;
+      info->used_solver = cfg->solver;
+    }
+    std::vector<double> r2, b2;
+    PM_TRY(pm_batch_true_res(c, b, x, res, r2, b2));
+    double worst = 0.0;
+    for (size_t k = 0; k < r2.size(); k++)
+      if (b2[k] > 0) worst = std::max(worst, sqrt(r2[k] / b2[k]));
+    info->true_rel = worst;
+    PM_CHECK_HIP(hipEventRecord(c->ev1, c->stream));
+    PM_CHECK_HIP(hipEventSynchronize(c->ev1));
+    float bms = 0.f;
+    PM_CHECK_HIP(hipEventElapsedTime(&bms, c->ev0, c->ev1));
+    info->ms = bms;
+    return PM_OK;
+  }
   if (cfg->precision == PM_PREC_FP64) {
     PM_TRY(krylov(c, cfg, n, b, x, c->d_m, cfg->tol, 0, info));
   } else if (cfg->precision == PM_PREC_FP32IR) {

@@ -4,6 +4,7 @@
 // porosity_form (Pform.f:969-1176) and update_res_v2 (Pload.f:483-541).
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "pm_device.h"
 
@@ -218,6 +219,8 @@ int pm_step_dev(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_info *in
     hipLaunchKernelGGL(k_predict, dim3(gn), dim3(tb), 0, c->stream, M);
     int lhs = 1, i = 1;
     double s0 = 0.0, s1 = 0.0;  // saved resid0 values (res0 / resu0, resp0)
+    const int nbox = pm_batch_nbox(c);
+    std::vector<double> bs0(nbox, 0.0), bs1(nbox, 0.0);  // the same, per box
     for (;;) {
       hipLaunchKernelGGL(k_copy, dim3(ge), dim3(tb), 0, c->stream, neq, c->d_b, c->d_b0);
       hipEventRecord(ea, c->stream);
@@ -225,9 +228,58 @@ int pm_step_dev(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_info *in
       hipEventRecord(eb, c->stream);
       ms_elem += elapsed(c, ea, eb);
       if (rc) break;
+      info->nlit = i;
+      if (nbox > 0) {  // every box runs its own nonlinear stop test
+        std::vector<double> ru2v, rp2v;
+        if ((rc = pm_batch_resnorms(c, c->d_b, ru2v, rp2v))) break;
+        std::vector<int> active(nbox, 0);
+        int nactive = 0;
+        for (int k = 0; k < nbox; k++) {
+          int ist = 0;
+          if (cfg->stop_crit == 1) {
+            const double res = sqrt(ru2v[k] + rp2v[k]);
+            if (i == 1) bs0[k] = res;
+            else if (i == 2) bs0[k] = fmax(bs0[k], res);
+            if (res / bs0[k] < cfg->nltol) ist = 2;
+          } else {
+            const double gu = sqrt(ru2v[k]), gp = sqrt(rp2v[k]);
+            if (i == 1) {
+              bs0[k] = gu;
+              bs1[k] = gp;
+            } else if (i == 2) {
+              bs0[k] = fmax(bs0[k], gu);
+              bs1[k] = fmax(bs1[k], gu);
+            }
+            if (gu / bs0[k] < cfg->nltol) ist++;
+            if (gp / bs1[k] < cfg->nltol) ist++;
+          }
+          active[k] = ist != 2;
+          nactive += active[k];
+        }
+        if (nactive == 0) break;
+        if ((rc = pm_batch_mask_rhs(c, c->d_b, active))) break;
+        pm_solve_info si;
+        rc = pm_solve_dev(c, cfg, c->d_b, c->d_x0, &si);
+        if (info->nsolve < 16) {
+          info->iters[info->nsolve] = si.iters;
+          info->used[info->nsolve] = si.used_solver;
+          info->true_rel[info->nsolve] = si.true_rel;
+        }
+        info->nsolve++;
+        ms_solve += si.ms;
+        if (rc) break;
+        M = march_args(c, dt);
+        hipLaunchKernelGGL(k_delta_update, dim3(gn), dim3(tb), 0, c->stream, M);
+        if (i >= cfg->maxnlit) {
+          rc = PM_ERR_NLIT;
+          break;
+        }
+        i++;
+        lhs = 0;
+        continue;
+      }
       double ru2, rp2;
       if ((rc = residual_norms(c, c->d_b, &ru2, &rp2))) break;
-      info->nlit = i;
       int istop = 0;
       if (cfg->stop_crit == 1) {
         const double res = sqrt(ru2 + rp2);

@@ -220,3 +220,45 @@ def box_counts(nx, ny, nz):
     nv = (nx + 1) * (ny + 1) * (nz + 1)
     nn = nv + nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz
     return {"numel": nx * ny * nz, "nnodev": nv, "nnode": nn, "dofs": 3 * nn + nv}
+
+
+def concat_decks(decks):
+    """One deck holding independent boxes side by side (node ids, material ids
+    offset per box), for the batching form.  All decks must share dt, gravity
+    and the load table.  Returns (deck, elem_off) with box b owning elements
+    elem_off[b]:elem_off[b+1]."""
+    first = decks[0]
+    ix, x, idv, e, el, elp = [], [], [], [], [], []
+    node_off = mat_off = 0
+    elem_off = [0]
+    for d in decks:
+        assert d.dt == first.dt and np.array_equal(d.load, first.load)
+        a = d.ix.copy()
+        a[:, :8] += node_off
+        a[:, NEN] += mat_off
+        ix.append(a)
+        x.append(d.x)
+        idv.append(d.idv)
+        e.append(d.e)
+        el.append(d.eload)
+        elp.append(d.eloadp)
+        node_off += d.nnodev
+        mat_off += d.e.shape[0]
+        elem_off.append(elem_off[-1] + d.numel)
+    deck = Deck(0, 0, 0, np.vstack(ix), np.vstack(x), np.vstack(idv), np.vstack(e),
+                np.vstack(el), np.vstack(elp), first.load, first.fload, first.gravity,
+                first.gravity_mod, first.dt, first.nsteps)
+    return deck, np.array(elem_off, dtype=np.int32)
+
+
+def batch_decks(nbox=256, seed=0, nmin=12, nmax=32, dt=0.1, nsteps=50):
+    """Independent boxes of n^3 cells, n ~ U[nmin, nmax] per box, each with its
+    own log-normal permeability (sigma ~ U[0.5, 1.5]) and E ~ U[5, 50] MPa."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, size=nbox)
+    sigmas = rng.uniform(0.5, 1.5, size=nbox)
+    seeds = rng.integers(0, 2**31 - 1, size=nbox)
+    decks = [heterogeneous_box_deck(int(n), seed=int(sd), sigma_lnk=float(sg), dt=dt,
+                                    nsteps=nsteps, corr_len=min(8.0, n / 2))
+             for n, sg, sd in zip(ns, sigmas, seeds)]
+    return decks

@@ -170,6 +170,21 @@ class MefSolver:
                     used=list(info.used[:n]), true_rel=list(info.true_rel[:n]),
                     ms_total=info.ms_total, ms_solve=info.ms_solve, ms_element=info.ms_element)
 
+    # -- batching form --------------------------------------------------------
+    def batch_define(self, elem_off):
+        self.elem_off = L.as_i32(elem_off)
+        self.nbox = len(self.elem_off) - 1
+        L.check(self.lib.pm_batch_define(self.ctx, self.nbox, L.ptr(self.elem_off)),
+                "pm_batch_define")
+
+    def batch_iters(self):
+        it = np.zeros(self.nbox, np.int32)
+        L.check(self.lib.pm_batch_iters(self.ctx, L.ptr(it)), "pm_batch_iters")
+        return it
+
+    def batch_graphs(self, on: bool):
+        L.check(self.lib.pm_batch_graphs(self.ctx, int(on)), "pm_batch_graphs")
+
     def matrix_info(self):
         neq, nb, nnz, ncols = C.c_int(), C.c_int(), C.c_int64(), C.c_int64()
         L.check(self.lib.pm_matrix_info(self.ctx, C.byref(neq), C.byref(nb), C.byref(nnz),
