@@ -185,6 +185,21 @@ int pm_batch_iters(pm_ctx *ctx, int *iters);
  * hipGraph; 0: launch them one by one. */
 int pm_batch_graphs(pm_ctx *ctx, int on);
 
+/* ---- adjoint form (no counterpart in the reference: it has no derivatives) ----
+ * Marches nsteps backward-Euler steps of size dt from the current state (as
+ * nsteps calls of pm_step), measuring the misfit
+ *   J = sum_n sum_i (u[sensor_dof[i]] after step n - target[n * nsens + i])^2
+ * where sensor_dof[i] = 4 * node + dof names a free dof of the nodal state
+ * (dof 3 = pressure), then solves the adjoint equations backwards in time
+ * and returns dJ/dlog(e(1)) (Young's modulus) in grad_logE and
+ * dJ/dlog(e(3)) (permeability) in grad_logk, one entry per material (numat;
+ * either may be NULL).  Linear elastic materials, one solve per step
+ * (tight solver tolerance); forward states are recomputed from checkpoints
+ * taken every ckpt_every steps.  Not available in the batching form. */
+int pm_adjoint(pm_ctx *ctx, int nsteps, double dt, const pm_solver_cfg *cfg, int nsens,
+               const int *sensor_dof, const double *target, int ckpt_every, double *J,
+               double *grad_logE, double *grad_logk);
+
 /* ---- measurement hooks (no counterpart in the reference) ----
  * Sizes of the device matrix: equations, row blocks (free nodes), stored
  * values (full matrix) and shared column indices. */

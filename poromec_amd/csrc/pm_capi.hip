@@ -698,6 +698,15 @@ extern "C" int pm_step(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_i
   return pm_step_dev(c, dt, cfg, info ? info : &local);
 }
 
+extern "C" int pm_adjoint(pm_ctx *c, int nsteps, double dt, const pm_solver_cfg *cfg, int nsens,
+                          const int *sensor_dof, const double *target, int ckpt_every, double *J,
+                          double *grad_logE, double *grad_logk) {
+  if (!c || !cfg) return PM_ERR_ARG;
+  PM_CHECK_HIP(hipSetDevice(c->device));
+  return pm_adjoint_dev(c, nsteps, dt, cfg, nsens, sensor_dof, target, ckpt_every, J, grad_logE,
+                        grad_logk);
+}
+
 extern "C" int pm_matrix_info(pm_ctx *c, int *neq, int *nblocks, int64_t *nnz, int64_t *ncols) {
   if (!c || !c->have_matrix) return PM_ERR_STATE;
   if (neq) *neq = c->A.neq;

@@ -189,3 +189,6 @@ int pm_batch_resnorms(pm_ctx *c, const double *b, std::vector<double> &ru2,
 int pm_batch_mask_rhs(pm_ctx *c, double *b, const std::vector<int> &active);
 int pm_batch_true_res(pm_ctx *c, const double *b, const double *x, double *res,
                       std::vector<double> &r2, std::vector<double> &b2);
+int pm_adjoint_dev(pm_ctx *c, int nsteps, double dt, const pm_solver_cfg *cfg, int nsens,
+                   const int *sensor_dof, const double *target, int ckpt_every, double *J,
+                   double *grad_logE, double *grad_logk);

@@ -170,6 +170,29 @@ class MefSolver:
                     used=list(info.used[:n]), true_rel=list(info.true_rel[:n]),
                     ms_total=info.ms_total, ms_solve=info.ms_solve, ms_element=info.ms_element)
 
+    # -- adjoint form -----------------------------------------------------------
+    def adjoint(self, nsteps: int, sensors, target, cfg: SolverConfig | None = None,
+                dt: float | None = None, ckpt_every: int = 8):
+        """March ``nsteps`` steps from the current state, measure
+        ``J = sum_n sum_i (u[node_i, dof_i] after step n - target[n, i])**2``
+        and return ``(J, dJ/dlog E, dJ/dlog k)`` per material, the gradients
+        with respect to ``log e[:, 0]`` and ``log e[:, 2]``.  ``sensors`` is a
+        list of ``(node, dof)`` pairs of free dofs (dof 3 = pressure).  The
+        state is left at the end of the march."""
+        cfg = cfg or SolverConfig()
+        dt = self.deck.dt if dt is None else dt
+        sens = np.asarray(sensors, dtype=np.int64).reshape(-1, 2)
+        sdof = L.as_i32(4 * sens[:, 0] + sens[:, 1])
+        tgt = L.as_f64(np.asarray(target, dtype=np.float64).reshape(nsteps, len(sdof)))
+        numat = self.deck.e.shape[0]
+        gE, gk = np.zeros(numat), np.zeros(numat)
+        J = C.c_double()
+        L.check(self.lib.pm_adjoint(self.ctx, int(nsteps), float(dt), C.byref(cfg.c()), len(sdof),
+                                    L.ptr(sdof), L.ptr(tgt), int(ckpt_every), C.byref(J),
+                                    L.ptr(gE), L.ptr(gk)), "pm_adjoint")
+        self.t += nsteps * dt
+        return J.value, gE, gk
+
     # -- batching form --------------------------------------------------------
     def batch_define(self, elem_off):
         self.elem_off = L.as_i32(elem_off)

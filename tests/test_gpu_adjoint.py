@@ -1,0 +1,138 @@
+"""The adjoint form on the GPU.
+
+The gradient of a sensor misfit over a march with respect to per-material
+log E and log permeability must match central finite differences of the
+same device march (relative error 1e-5 along random and coordinate
+directions, Krylov tolerance 1e-13 so that each step is one exact solve).
+Checkpoint spacing must not change J, the gradients or the final state, and
+the torch.autograd wrapper must hand back the library's gradient."""
+import numpy as np
+import pytest
+
+from poromec_amd import problem as P
+
+pytestmark = pytest.mark.gpu
+
+NSTEPS = 6
+
+
+def _deck(gravity: bool):
+    d = P.heterogeneous_box_deck(3, seed=5, sigma_lnk=0.8, corr_len=2.0, dt=0.05, nsteps=NSTEPS)
+    if gravity:
+        d.gravity = np.array([0.0, 0.0, -1.0])
+        d.e[:, 5] = 2000.0
+        d.e[:, 6] = 1000.0
+    return d
+
+
+def _cfg():
+    from poromec_amd.solver import SolverConfig
+    return SolverConfig(tol=1e-13, maxit=200000)
+
+
+def _sensors(solver, rng, npress=5, ndisp=2):
+    _, idm, _ = solver.numbering()
+    nv = solver.deck.nnodev
+    pfree = np.nonzero(idm[:nv, 3] > 0)[0]
+    ufree = np.nonzero(idm[:, 2] > 0)[0]
+    ps = rng.choice(pfree, size=npress, replace=False)
+    us = rng.choice(ufree, size=ndisp, replace=False)
+    return [(int(n), 3) for n in ps] + [(int(n), 2) for n in us]
+
+
+def _misfit(gravity):
+    from poromec_amd.adjoint import MarchMisfit
+    from poromec_amd.solver import MefSolver
+    deck = _deck(gravity)
+    rng = np.random.default_rng(11)
+    probe = MefSolver(deck)
+    sens = _sensors(probe, rng)
+    # targets: the march of a perturbed material set, so the misfit is O(1)
+    e = deck.e.copy()
+    deck.e = e * np.where(np.arange(e.shape[1]) == 2, 1.5, 1.0)
+    probe.set_materials(deck)
+    deck.e = e
+    obs = np.zeros((NSTEPS, len(sens)))
+    for n in range(NSTEPS):
+        probe.step(_cfg())
+        u = probe.state()["u"]
+        obs[n] = [u[a, b] for a, b in sens]
+    probe.close()
+    return MarchMisfit(deck, sens, obs, nsteps=NSTEPS, cfg=_cfg(), ckpt_every=4)
+
+
+@pytest.mark.parametrize("gravity", [False, True])
+def test_adjoint_matches_finite_differences(gravity):
+    m = _misfit(gravity)
+    lE, lk = (t.numpy() for t in m.params())
+    J, gE, gk = m._evaluate(lE, lk)
+    assert J > 0 and np.all(np.isfinite(gE)) and np.all(np.isfinite(gk))
+    rng = np.random.default_rng(3)
+    h = 1e-4
+    dirs = [rng.standard_normal(2 * len(lE)) for _ in range(2)]
+    for ci in (int(np.argmax(np.abs(gE))), len(lE) + int(np.argmax(np.abs(gk)))):
+        v = np.zeros(2 * len(lE))
+        v[ci] = 1.0
+        dirs.append(v)
+    g = np.concatenate([gE, gk])
+    for v in dirs:
+        vE, vk = v[: len(lE)], v[len(lE):]
+        jp = m._evaluate(lE + h * vE, lk + h * vk)[0]
+        jm = m._evaluate(lE - h * vE, lk - h * vk)[0]
+        fd = (jp - jm) / (2 * h)
+        ad = float(g @ v)
+        assert abs(fd - ad) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(g).sum()), (fd, ad)
+    m.close()
+
+
+def test_checkpoint_spacing_is_invisible():
+    from poromec_amd.solver import MefSolver
+    deck = _deck(True)
+    runs = []
+    for ck in (1, 4, NSTEPS):
+        s = MefSolver(deck)
+        sens = _sensors(s, np.random.default_rng(7))
+        tgt = np.random.default_rng(2).standard_normal((NSTEPS, len(sens))) * 1e-4
+        J, gE, gk = s.adjoint(NSTEPS, sens, tgt, _cfg(), ckpt_every=ck)
+        runs.append((J, gE, gk, s.state()["u"].copy()))
+        s.close()
+    plain = MefSolver(deck)
+    for _ in range(NSTEPS):
+        plain.step(_cfg())
+    u_plain = plain.state()["u"]
+    plain.close()
+    for J, gE, gk, u in runs[1:]:
+        assert J == pytest.approx(runs[0][0], rel=1e-13)
+        np.testing.assert_allclose(gE, runs[0][1], rtol=1e-11, atol=1e-14 * np.abs(gE).max())
+        np.testing.assert_allclose(gk, runs[0][2], rtol=1e-11, atol=1e-14 * np.abs(gk).max())
+    for *_, u in runs:
+        assert np.array_equal(u, u_plain)
+
+
+def test_autograd_wrapper():
+    import torch
+    m = _misfit(False)
+    lE, lk = m.params()
+    lE.requires_grad_(True)
+    lk.requires_grad_(True)
+    J = m(lE, lk)
+    (2.0 * J).backward()
+    J2, gE, gk = m._evaluate(lE.detach().numpy(), lk.detach().numpy())
+    assert float(J) == J2
+    torch.testing.assert_close(lE.grad, torch.from_numpy(2.0 * gE))
+    torch.testing.assert_close(lk.grad, torch.from_numpy(2.0 * gk))
+    m.close()
+
+
+def test_adjoint_rejects_bad_sensors():
+    from poromec_amd._lib import PmError
+    from poromec_amd.solver import MefSolver
+    deck = _deck(False)
+    s = MefSolver(deck)
+    _, idm, _ = s.numbering()
+    fixed = int(np.nonzero(idm[:, 2] == 0)[0][0])
+    with pytest.raises(PmError):
+        s.adjoint(2, [(fixed, 2)], np.zeros((2, 1)), _cfg())
+    with pytest.raises(PmError):
+        s.adjoint(2, [(s.nnode, 0)], np.zeros((2, 1)), _cfg())
+    s.close()
