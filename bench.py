@@ -63,6 +63,7 @@ def run_form(solver, cfg, warmup, steps, keep_states=False):
         if keep_states is not None:
             st = solver.state()
             states.append((st["u"].copy(), st["dp"].copy()))
+    solver.spmv_timing(True)  # HIP events around every SpMV of the timed steps
     t0 = time.perf_counter()
     for k in range(steps):
         info = solver.step(cfg)
@@ -72,10 +73,11 @@ def run_form(solver, cfg, warmup, steps, keep_states=False):
             # state readback happens after the step returned; excluded below
             pass
     wall = time.perf_counter() - t0
+    spmv = solver.spmv_timing(False)
     if keep_states is not None and steps:
         st = solver.state()
         states.append((st["u"].copy(), st["dp"].copy()))
-    return wall, infos, states
+    return wall, infos, states, spmv
 
 
 def cpu_baseline(n_sample, n_bench_numel, bench_nnz, gpu_iters_per_step, seed=0):
@@ -149,6 +151,8 @@ def main():
 
     if args.spmv_only:
         solver.pform(2, True, False, deck.dt, deck.dt)
+        solver.tune_spmv(False, reps=5)
+        solver.tune_spmv(True, reps=5)
         ms64 = solver.bench_spmv(False, args.spmv_reps)
         ms32 = solver.bench_spmv(True, args.spmv_reps)
         copy_bytes = 1 << 30
@@ -160,11 +164,21 @@ def main():
                           "copy_GBps": 2 * copy_bytes / msc / 1e6, **info}), flush=True)
         return
 
+    # SpMV shape tuner on this matrix (lanes per node block x columns in flight)
+    solver.pform(2, True, False, deck.dt, deck.dt)
+    shapes = {}
+    for use32, name in ((False, "fp64"), (True, "fp32")):
+        best, ms = solver.tune_spmv(use32, reps=5)
+        shapes[name] = {"best": best, "ms": [round(v, 4) for v in ms]}
+        log(f"SpMV shapes {name}: " + ", ".join(f"{v:.3f}" for v in ms) + f" ms -> shape {best}")
+
     cfg64 = SolverConfig(precision="fp64", tol=1e-10)
     cfg32 = SolverConfig(precision="fp32ir", tol=1e-10, ir_inner_tol=1e-6)
 
-    wall64, infos64, st64 = run_form(solver, cfg64, args.warmup, args.steps, keep_states=True)
-    wall32, infos32, st32 = run_form(solver, cfg32, args.warmup, args.steps, keep_states=True)
+    wall64, infos64, st64, live64 = run_form(solver, cfg64, args.warmup, args.steps,
+                                             keep_states=True)
+    wall32, infos32, st32, live32 = run_form(solver, cfg32, args.warmup, args.steps,
+                                             keep_states=True)
 
     # error of the FP32+IR histories against the FP64 form, step by step
     nv = deck.nnodev
@@ -175,11 +189,17 @@ def main():
         err_u = max(err_u, np.abs(u32[:, :3] - u64[:, :3]).max() / su)
         err_p = max(err_p, np.abs(u32[:nv, 3] - u64[:nv, 3]).max() / sp)
 
-    ms64 = solver.bench_spmv(False, args.spmv_reps)
-    ms32 = solver.bench_spmv(True, args.spmv_reps)
+    # SpMV kernel time inside the timed steps (HIP events on the library's
+    # stream around every solver SpMV), per value precision
     b64, b32 = spmv_bytes(info, 8), spmv_bytes(info, 4)
+    ms64 = live64["fp64_ms"] / max(live64["fp64_n"], 1)
+    ms32 = live32["fp32_ms"] / max(live32["fp32_n"], 1)
     gbps64, gbps32 = b64 / ms64 / 1e6, b32 / ms32 / 1e6
-    log(f"SpMV fp64 {ms64:.3f} ms ({gbps64:.0f} GB/s), fp32 {ms32:.3f} ms ({gbps32:.0f} GB/s)")
+    log(f"SpMV in the timed steps: fp64 {ms64:.3f} ms x {live64['fp64_n']} ({gbps64:.0f} GB/s), "
+        f"fp32 {ms32:.3f} ms x {live32['fp32_n']} ({gbps32:.0f} GB/s)")
+    iso64 = solver.bench_spmv(False, args.spmv_reps)
+    iso32 = solver.bench_spmv(True, args.spmv_reps)
+    log(f"SpMV back to back: fp64 {iso64:.3f} ms, fp32 {iso32:.3f} ms")
 
     steps = args.steps
     it64 = sum(sum(i["iters"]) for i in infos64) / max(steps, 1)
@@ -226,7 +246,10 @@ def main():
                    "max_rel_err_vs_fp64": {"u": err_u, "p": err_p}},
         "speedup_fp32ir_vs_fp64": ms_step64 / ms_step32,
         "spmv": {"fp64_ms": ms64, "fp32_ms": ms32, "fp64_GBps": gbps64, "fp32_GBps": gbps32,
+                 "fp64_launches": live64["fp64_n"], "fp32_launches": live32["fp32_n"],
                  "fp64_bytes": b64, "fp32_bytes": b32,
+                 "back_to_back_fp64_ms": iso64, "back_to_back_fp32_ms": iso32,
+                 "tuned_shapes": shapes,
                  "csr_equivalent_fp64_GBps": csr_bytes(info) / ms64 / 1e6},
         "roofline": {"bound": "hbm", "kernel": "k_spmv<float> (FP32 node-block SpMV)",
                      "achieved": gbps32, "peak": HBM_PEAK_GBPS, "unit": "GB/s",

@@ -171,6 +171,18 @@ int pm_set_vectors(pm_ctx *ctx, const double *b0, const double *b, const double 
  * porosity_form, update_res_v2. */
 int pm_step(pm_ctx *ctx, double dt, const pm_solver_cfg *cfg, pm_step_info *info);
 
+/* ---- reference summation order (no counterpart in the reference: it is the
+ * reference's own arithmetic order) ----
+ * on != 0: every dot product and norm of the solvers and of the residual
+ * test is summed sequentially in equation order (dot_par, Matvec.f:1714-1743,
+ * one rank), and the SpMV sums each row as matvec_csrc_sym_pm does (diagonal,
+ * lower columns ascending, transposed terms ascending; Matvec.f:1504-1595).
+ * With a matrix whose two triangles hold identical values (pm_values_upload)
+ * the solvers then reproduce the reference's iterates bit for bit.  The
+ * reductions run on one thread: for verification, not for speed.  Not used
+ * by the batching form. */
+int pm_set_ordered(pm_ctx *ctx, int on);
+
 /* ---- batching form (no counterpart in the reference: the reference runs one
  * problem per process) ----
  * The mesh given to pm_mesh_setup holds nbox disconnected boxes, box b owning
@@ -208,6 +220,23 @@ int pm_matrix_info(pm_ctx *ctx, int *neq, int *nblocks, int64_t *nnz, int64_t *n
  * use32 != 0) over reps launches, timed with HIP events on the library's
  * stream. */
 int pm_bench_spmv(pm_ctx *ctx, int use32, int reps, double *ms_per_launch);
+/* SpMV shape tuner (no counterpart in the reference): times each of the
+ * PM_SPMV_SHAPES (lanes per node block, columns in flight per lane) shapes
+ * on the current matrix over `reps` launches, keeps the fastest for FP64
+ * (use32 = 0) or FP32 values, and returns its index and, if ms_per_shape is
+ * not NULL, every shape's average launch time.  The batching form tunes its
+ * own kernel.  Shapes sum in different orders: results move at the rounding
+ * level when the shape changes, and stay fixed until the next change. */
+#define PM_SPMV_SHAPES 6
+int pm_tune_spmv(pm_ctx *ctx, int use32, int reps, int *best, double *ms_per_shape);
+/* Selects SpMV shape `shape` (0 .. PM_SPMV_SHAPES-1) without timing. */
+int pm_set_spmv_shape(pm_ctx *ctx, int use32, int shape);
+/* Live SpMV timing: with on != 0 every SpMV the single-system solvers launch
+ * is bracketed by HIP events on the library's stream (launches skipped after
+ * convergence are not counted).  stats (may be NULL) receives what was
+ * accumulated since the previous call -- {ms FP64, launches FP64, ms FP32,
+ * launches FP32} -- and the accumulators restart from zero. */
+int pm_spmv_timing(pm_ctx *ctx, int on, double *stats);
 /* Average device time of a plain copy kernel over `bytes` bytes (reads
  * bytes, writes bytes): the reference point for HBM traffic counters. */
 int pm_bench_copy(pm_ctx *ctx, int64_t bytes, int reps, double *ms_per_launch);

@@ -91,6 +91,10 @@ def lib():
     L.pm_batch_define.argtypes = [_vp, C.c_int, _vp]
     L.pm_batch_iters.argtypes = [_vp, _vp]
     L.pm_batch_graphs.argtypes = [_vp, C.c_int]
+    L.pm_set_ordered.argtypes = [_vp, C.c_int]
+    L.pm_spmv_timing.argtypes = [_vp, C.c_int, _vp]
+    L.pm_tune_spmv.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_int), _vp]
+    L.pm_set_spmv_shape.argtypes = [_vp, C.c_int, C.c_int]
     L.pm_adjoint.argtypes = [_vp, C.c_int, C.c_double, C.POINTER(SolverCfg), C.c_int, _vp, _vp,
                              C.c_int, C.POINTER(C.c_double), _vp, _vp]
     L.pm_matrix_info.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64),

@@ -23,6 +23,9 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # no packed FP32 instructions in any device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # every product and sum rounded on its own, as in the reference's Fortran
+    # (no fused multiply-add contraction)
+    "-ffp-contract=off",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
 ]
 

@@ -11,9 +11,11 @@
 // iterations are captured once in a hipGraph and replayed.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "pm_functors.h"
+#include "pm_spmv.h"
 
 struct PmBatch {
   int nbox = 0;
@@ -27,7 +29,7 @@ struct PmBatch {
   std::vector<int> last_iters;
   // hipGraph cache of one chunk of iterations
   hipGraphExec_t graph = nullptr;
-  uintptr_t graph_key[6] = {0, 0, 0, 0, 0, 0};
+  uintptr_t graph_key[7] = {0, 0, 0, 0, 0, 0};
   int use_graph = 1;
   ~PmBatch() {
     if (graph) (void)hipGraphExecDestroy(graph);
@@ -53,55 +55,6 @@ struct BatchArgs {
   int *ndone;
 };
 
-// reduction over the workgroups [gbase, gbase + gcount) of one box
-template <int NV>
-__device__ __forceinline__ bool pm_group_reduce(const double *v, double *partials,
-                                                unsigned int *counter, int gbase, int gcount,
-                                                double *out) {
-  constexpr int NS = NV > 0 ? NV : 1;
-  __shared__ double sh[NS][PM_WAVES];
-  __shared__ int amlast;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < NV; i++) {
-    double w = pm_wave_sum(v[i]);
-    if (lane == 0) sh[i][wave] = w;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < NV; i++) {
-      double acc = sh[i][0];
-      for (int w = 1; w < PM_WAVES; w++) acc += sh[i][w];
-      partials[(size_t)blockIdx.x * NS + i] = acc;
-    }
-    __threadfence();
-    unsigned int t = atomicAdd(counter, 1u);
-    amlast = (t == (unsigned)gcount - 1u);
-  }
-  __syncthreads();
-  if (!amlast) return false;
-  __threadfence();
-#pragma unroll
-  for (int i = 0; i < NV; i++) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < gcount; b += PM_WG)
-      acc += __builtin_nontemporal_load(&partials[(size_t)(gbase + b) * NS + i]);
-    acc = pm_wave_sum(acc);
-    __syncthreads();
-    if (lane == 0) sh[i][wave] = acc;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < NV; i++) {
-    double acc = sh[i][0];
-    for (int w = 1; w < PM_WAVES; w++) acc += sh[i][w];
-    out[i] = acc;
-  }
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
-}
-
 template <class Epi>
 struct CountDone {  // epilogue wrapper: counts boxes that finish
   Epi e;
@@ -113,7 +66,7 @@ struct CountDone {  // epilogue wrapper: counts boxes that finish
   }
 };
 
-template <typename VT, int NV, class Epi>
+template <typename VT, int LANES, int UNR, int NV>
 __global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, const int *__restrict__ fr,
                                                   const int *__restrict__ nr,
                                                   const int64_t *__restrict__ colptr,
@@ -122,61 +75,45 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, const int *__rest
                                                   const VT *__restrict__ val,
                                                   const double *__restrict__ x,
                                                   double *__restrict__ y,
-                                                  const double *__restrict__ w, PmScalars *S,
-                                                  Epi epi) {
+                                                  const double *__restrict__ w,
+                                                  const PmScalars *S) {
   const int box = B.wg_box[blockIdx.x];
-  PmScalars *Sb = S + box;
+  const PmScalars *Sb = S + box;
   if (Sb->done) return;
+  constexpr int GROUPS = PM_WG / LANES;
   const int gb = B.wg0[box], gc = B.wg0[box + 1] - gb;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
   const int64_t nb0 = B.blk0[box], nblk = B.blk0[box + 1] - nb0;
   int64_t k0, k1;
   pm_chunk(nblk, blockIdx.x - gb, gc, k0, k1);
   double d0 = 0.0, d1 = 0.0;
-  for (int64_t kk = k0 + wave; kk < k1; kk += PM_WAVES) {
+  for (int64_t kk = k0 + grp; kk < k1; kk += GROUPS) {
     const int64_t k = nb0 + kk;
     const int64_t cp = colptr[k];
     const int c = (int)(colptr[k + 1] - cp);
-    const int64_t vp = valptr[k];
     const int n = nr[k];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (int j = lane; j < c; j += 64) {
-      const int col = __builtin_nontemporal_load(ncol + cp + j);
-      const double xv = x[col];
-      const VT *v = val + vp + j;
-      a0 += (double)__builtin_nontemporal_load(v) * xv;
-      if (n > 1) a1 += (double)__builtin_nontemporal_load(v + c) * xv;
-      if (n > 2) a2 += (double)__builtin_nontemporal_load(v + 2 * c) * xv;
-      if (n > 3) a3 += (double)__builtin_nontemporal_load(v + 3 * c) * xv;
-    }
-    a0 = pm_wave_sum(a0);
-    if (n > 1) a1 = pm_wave_sum(a1);
-    if (n > 2) a2 = pm_wave_sum(a2);
-    if (n > 3) a3 = pm_wave_sum(a3);
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    pm_spmv_block<VT, LANES, UNR>(lane, cp, c, valptr[k], n, ncol, val, x, a);
     if (lane == 0) {
       const int f = fr[k];
-      const double r[4] = {a0, a1, a2, a3};
       for (int q = 0; q < n; q++) {
-        y[f + q] = r[q];
-        if (NV >= 1) d0 += r[q] * w[f + q];
-        if (NV >= 2) d1 += r[q] * r[q];
+        y[f + q] = a[q];
+        if (NV >= 1) d0 += a[q] * w[f + q];
+        if (NV >= 2) d1 += a[q] * a[q];
       }
     }
   }
   if (NV > 0) {
-    double v[2] = {d0, d1}, out[2];
-    if (pm_group_reduce<NV>(v, B.partials, B.counters + box, gb, gc, out) && threadIdx.x == 0) {
-      for (int i = 0; i < NV; i++) Sb->red[i] = out[i];
-      epi(Sb, out);
-    }
+    double v[2] = {d0, d1};
+    pm_block_partials<NV>(v, B.partials);
   }
 }
 
-template <int NV, class Body, class Epi>
-__global__ __launch_bounds__(PM_WG) void k_vec_b(BatchArgs B, PmScalars *S, Body body, Epi epi,
+template <int NV, class Body>
+__global__ __launch_bounds__(PM_WG) void k_vec_b(BatchArgs B, const PmScalars *S, Body body,
                                                  int ignore_done) {
   const int box = B.wg_box[blockIdx.x];
-  PmScalars *Sb = S + box;
+  const PmScalars *Sb = S + box;
   if (!ignore_done && Sb->done) return;
   const int gb = B.wg0[box], gc = B.wg0[box + 1] - gb;
   const int64_t r0 = B.eq0[box], nrow = B.eq0[box + 1] - r0;
@@ -186,8 +123,21 @@ __global__ __launch_bounds__(PM_WG) void k_vec_b(BatchArgs B, PmScalars *S, Body
   for (int i = 0; i < (NV > 0 ? NV : 1); i++) acc[i] = 0.0;
   const PmScalars &s = *Sb;
   for (int64_t i = a + threadIdx.x; i < e; i += PM_WG) body(r0 + i, s, acc);
+  pm_block_partials<NV>(acc, B.partials);
+}
+
+// one workgroup per box: sums the box's workgroup partials of the kernel
+// before it (fixed order) and runs the box's epilogue
+template <int NV, class Epi>
+__global__ __launch_bounds__(PM_WG) void k_finish_b(BatchArgs B, PmScalars *S, Epi epi,
+                                                    int ignore_done) {
+  const int box = blockIdx.x;
+  PmScalars *Sb = S + box;
+  if (!ignore_done && Sb->done) return;
+  const int gb = B.wg0[box], gc = B.wg0[box + 1] - gb;
   double out[NV > 0 ? NV : 1];
-  if (pm_group_reduce<NV>(acc, B.partials, B.counters + box, gb, gc, out) && threadIdx.x == 0) {
+  pm_sum_partials<NV>(gc, B.partials + (size_t)gb * NV, out);
+  if (threadIdx.x == 0) {
     for (int i = 0; i < NV; i++) Sb->red[i] = out[i];
     epi(Sb, out);
   }
@@ -214,24 +164,58 @@ static BatchArgs batch_args(PmBatch *B) {
 }
 
 template <int NV, class Epi>
-static void bspmv(pm_ctx *c, int use32, const double *x, double *y, const double *w, Epi epi) {
+static void bfinish(pm_ctx *c, Epi epi, int ignore_done) {
+  if (NV == 0 && std::is_same<Epi, NoEpi>::value) return;
+  PmBatch *B = c->batch;
+  hipLaunchKernelGGL((k_finish_b<NV, Epi>), dim3(B->nbox), dim3(PM_WG), 0, c->stream,
+                     batch_args(B), B->d_S, epi, ignore_done);
+}
+
+template <typename VT, int NV>
+static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, double *y,
+                        const double *w) {
   PmBatch *B = c->batch;
   const PmDevMatrix &A = c->A;
-  if (use32)
-    hipLaunchKernelGGL((k_spmv_b<float, NV, Epi>), dim3(B->G), dim3(PM_WG), 0, c->stream,
-                       batch_args(B), A.fr, A.nr, A.colptr, A.valptr, A.ncol, A.val32, x, y, w,
-                       B->d_S, epi);
-  else
-    hipLaunchKernelGGL((k_spmv_b<double, NV, Epi>), dim3(B->G), dim3(PM_WG), 0, c->stream,
-                       batch_args(B), A.fr, A.nr, A.colptr, A.valptr, A.ncol, A.val, x, y, w,
-                       B->d_S, epi);
+#define PM_BSPMV_CASE(id, L, U)                                                                 \
+  case id:                                                                                      \
+    hipLaunchKernelGGL((k_spmv_b<VT, L, U, NV>), dim3(B->G), dim3(PM_WG), 0, c->stream,        \
+                       batch_args(B), A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w,    \
+                       B->d_S);                                                                 \
+    break;
+  switch (cfg) {
+    PM_BSPMV_CASE(0, 64, 1)
+    PM_BSPMV_CASE(1, 64, 2)
+    PM_BSPMV_CASE(2, 64, 4)
+    PM_BSPMV_CASE(3, 32, 2)
+    PM_BSPMV_CASE(4, 32, 4)
+    default: PM_BSPMV_CASE(5, 16, 4)
+  }
+#undef PM_BSPMV_CASE
+}
+
+template <int NV, class Epi>
+static void bspmv(pm_ctx *c, int use32, const double *x, double *y, const double *w, Epi epi) {
+  const int cfg = c->spmv_cfg[use32 ? 1 : 0];
+  if (use32) bspmv_shape<float, NV>(c, cfg, c->A.val32, x, y, w);
+  else bspmv_shape<double, NV>(c, cfg, c->A.val, x, y, w);
+  if (NV > 0) bfinish<NV>(c, epi, 0);
+}
+
+// plain batched product y = A x (every box), for the shape tuner
+int pm_batch_spmv_dev(pm_ctx *c, const double *x, double *y, int use32) {
+  PmBatch *B = c->batch;
+  PM_CHECK_HIP(hipMemsetAsync(B->d_S, 0, sizeof(PmScalars) * B->nbox, c->stream));
+  bspmv<0>(c, use32, x, y, nullptr, NoEpi{});
+  PM_CHECK_HIP(hipGetLastError());
+  return PM_OK;
 }
 
 template <int NV, class Body, class Epi>
 static void bvec(pm_ctx *c, Body body, Epi epi, int ignore_done = 0) {
   PmBatch *B = c->batch;
-  hipLaunchKernelGGL((k_vec_b<NV, Body, Epi>), dim3(B->G), dim3(PM_WG), 0, c->stream,
-                     batch_args(B), B->d_S, body, epi, ignore_done);
+  hipLaunchKernelGGL((k_vec_b<NV, Body>), dim3(B->G), dim3(PM_WG), 0, c->stream, batch_args(B),
+                     B->d_S, body, ignore_done);
+  bfinish<NV>(c, epi, ignore_done);
 }
 
 static int breset(pm_ctx *c) {
@@ -288,8 +272,9 @@ int pm_batch_krylov(pm_ctx *c, int solver, int maxit, double tol, const double *
   PM_TRY(bpoll(c, &ndone));
   const int chunk = 16;
   // graph of one chunk: kernels only, fixed pointers for this (solver, precision, x, m)
-  const uintptr_t key[6] = {(uintptr_t)solver, (uintptr_t)use32, (uintptr_t)x, (uintptr_t)m,
-                            (uintptr_t)c->d_wk[0], (uintptr_t)c->A.val32};
+  const uintptr_t key[7] = {(uintptr_t)solver, (uintptr_t)use32, (uintptr_t)x, (uintptr_t)m,
+                            (uintptr_t)c->d_wk[0], (uintptr_t)c->A.val32,
+                            (uintptr_t)c->spmv_cfg[use32 ? 1 : 0]};
   if (B->use_graph && (B->graph == nullptr || memcmp(key, B->graph_key, sizeof(key)) != 0)) {
     if (B->graph) (void)hipGraphExecDestroy(B->graph);
     B->graph = nullptr;

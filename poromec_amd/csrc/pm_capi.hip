@@ -6,6 +6,7 @@
 #include <mutex>
 
 #include "pm_device.h"
+#include "pm_spmv.h"
 
 static thread_local char g_last_error[512] = "";
 
@@ -52,6 +53,7 @@ static void free_matrix(PmDevMatrix &A) {
 }
 
 pm_ctx::~pm_ctx() {
+  for (hipEvent_t e : tev) (void)hipEventDestroy(e);
   if (batch) pm_batch_free(batch);
   FREE(d_eq_box);
   free_matrix(A);
@@ -698,6 +700,12 @@ extern "C" int pm_step(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_i
   return pm_step_dev(c, dt, cfg, info ? info : &local);
 }
 
+extern "C" int pm_set_ordered(pm_ctx *c, int on) {
+  if (!c) return PM_ERR_ARG;
+  c->ordered = on != 0;
+  return PM_OK;
+}
+
 extern "C" int pm_adjoint(pm_ctx *c, int nsteps, double dt, const pm_solver_cfg *cfg, int nsens,
                           const int *sensor_dof, const double *target, int ckpt_every, double *J,
                           double *grad_logE, double *grad_logk) {
@@ -705,6 +713,20 @@ extern "C" int pm_adjoint(pm_ctx *c, int nsteps, double dt, const pm_solver_cfg 
   PM_CHECK_HIP(hipSetDevice(c->device));
   return pm_adjoint_dev(c, nsteps, dt, cfg, nsens, sensor_dof, target, ckpt_every, J, grad_logE,
                         grad_logk);
+}
+
+extern "C" int pm_spmv_timing(pm_ctx *c, int on, double *stats) {
+  if (!c) return PM_ERR_ARG;
+  if (stats) {
+    stats[0] = c->spmv_ms[0];
+    stats[1] = (double)c->spmv_n[0];
+    stats[2] = c->spmv_ms[1];
+    stats[3] = (double)c->spmv_n[1];
+  }
+  c->spmv_ms[0] = c->spmv_ms[1] = 0.0;
+  c->spmv_n[0] = c->spmv_n[1] = 0;
+  c->time_spmv = on != 0;
+  return PM_OK;
 }
 
 extern "C" int pm_matrix_info(pm_ctx *c, int *neq, int *nblocks, int64_t *nnz, int64_t *ncols) {
@@ -732,6 +754,60 @@ extern "C" int pm_bench_spmv(pm_ctx *c, int use32, int reps, double *ms_per_laun
   float ms = 0.f;
   PM_CHECK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   *ms_per_launch = ms / reps;
+  return PM_OK;
+}
+
+// The SpMV shape tuner: times every (LANES, UNR) shape on the current matrix
+// and keeps the fastest for that value precision.  Shapes sum in different
+// orders, so results move at the rounding level when the shape changes; the
+// choice stays fixed until the next call.
+extern "C" int pm_tune_spmv(pm_ctx *c, int use32, int reps, int *best, double *ms_per_shape) {
+  if (!c || reps <= 0) return PM_ERR_ARG;
+  if (!c->have_matrix || !c->A.values_valid) return PM_ERR_STATE;
+  PM_CHECK_HIP(hipSetDevice(c->device));
+  if (use32 && !c->A.have32) PM_TRY(pm_make_val32(c));
+  PM_TRY(pm_ensure_workspace(c, c->A.neq));
+  double *x = c->d_wk[0], *y = c->d_wk[1];
+  PM_CHECK_HIP(hipMemsetAsync(x, 0, sizeof(double) * c->A.neq, c->stream));
+  const int p = use32 ? 1 : 0, keep = c->spmv_cfg[p], ordered = c->ordered;
+  c->ordered = 0;
+  int arg = -1;
+  double tbest = 0.0;
+  for (int k = 0; k < PM_SPMV_NCFG; k++) {
+    c->spmv_cfg[p] = k;
+    auto spmv = [&](const double *a, double *b) {
+      return c->batch ? pm_batch_spmv_dev(c, a, b, use32) : pm_spmv_dev(c, a, b, use32);
+    };
+    int rc = spmv(x, y);  // warm-up
+    if (rc == PM_OK) rc = hipEventRecord(c->ev0, c->stream) == hipSuccess ? PM_OK : PM_ERR_HIP;
+    for (int r = 0; r < reps && rc == PM_OK; r++) rc = spmv((r & 1) ? y : x, (r & 1) ? x : y);
+    float ms = 0.f;
+    if (rc == PM_OK && (hipEventRecord(c->ev1, c->stream) != hipSuccess ||
+                        hipEventSynchronize(c->ev1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess))
+      rc = PM_ERR_HIP;
+    if (rc != PM_OK) {
+      c->spmv_cfg[p] = keep;
+      c->ordered = ordered;
+      return rc;
+    }
+    const double per = ms / reps;
+    if (ms_per_shape) ms_per_shape[k] = per;
+    if (arg < 0 || per < tbest) {
+      arg = k;
+      tbest = per;
+    }
+  }
+  c->ordered = ordered;
+  c->spmv_cfg[p] = arg;
+  c->spmv_tuned[p] = 1;
+  if (best) *best = arg;
+  return PM_OK;
+}
+
+extern "C" int pm_set_spmv_shape(pm_ctx *c, int use32, int shape) {
+  if (!c || shape < 0 || shape >= PM_SPMV_NCFG) return PM_ERR_ARG;
+  c->spmv_cfg[use32 ? 1 : 0] = shape;
   return PM_OK;
 }
 

@@ -36,55 +36,55 @@ __device__ __forceinline__ int pm_xcd_block(int b, int g) {
   return (b & 7) * (g >> 3) + (b >> 3);
 }
 
-// Deterministic grid reduction of NV values (NV may be 0: only elects the
-// last workgroup): workgroup partials at fixed slots, the last workgroup to
-// arrive sums them in a fixed order.  Returns true in every thread of the
-// last workgroup, with the sums in out[0..NV).
+// Workgroup partial sums of NV values, one slot per workgroup (fixed order
+// inside the workgroup).  No fence and no atomic: the partials are summed by
+// a later kernel on the same stream, after the kernel boundary has made them
+// visible on every XCD (a device-scope fence per workgroup would write back
+// the XCD's L2 each time).
 template <int NV>
-__device__ __forceinline__ bool pm_grid_reduce(const double *v, double *partials,
-                                               unsigned int *counter, double *out) {
-  constexpr int NS = NV > 0 ? NV : 1;
-  __shared__ double sh[NS][PM_WAVES];
-  __shared__ int amlast;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ __forceinline__ void pm_block_partials(const double *v, double *partials) {
+  if constexpr (NV > 0) {
+    __shared__ double sh[NV][PM_WAVES];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int i = 0; i < NV; i++) {
-    double w = pm_wave_sum(v[i]);
-    if (lane == 0) sh[i][wave] = w;
+    for (int i = 0; i < NV; i++) {
+      double w = pm_wave_sum(v[i]);
+      if (lane == 0) sh[i][wave] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        double acc = sh[i][0];
+        for (int w = 1; w < PM_WAVES; w++) acc += sh[i][w];
+        partials[(size_t)blockIdx.x * NV + i] = acc;
+      }
+    }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
+}
+
+// Sum of the partials of nblk workgroups in a fixed order, by one workgroup;
+// every thread receives the sums.
+template <int NV>
+__device__ __forceinline__ void pm_sum_partials(int nblk, const double *partials, double *out) {
+  if constexpr (NV > 0) {
+    __shared__ double sh[NV][PM_WAVES];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      double acc = 0.0;
+      for (int b = threadIdx.x; b < nblk; b += PM_WG) acc += partials[(size_t)b * NV + i];
+      acc = pm_wave_sum(acc);
+      if (lane == 0) sh[i][wave] = acc;
+    }
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < NV; i++) {
       double acc = sh[i][0];
       for (int w = 1; w < PM_WAVES; w++) acc += sh[i][w];
-      partials[(size_t)blockIdx.x * NS + i] = acc;
+      out[i] = acc;
     }
-    __threadfence();
-    unsigned int t = atomicAdd(counter, 1u);
-    amlast = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!amlast) return false;
-  __threadfence();
-#pragma unroll
-  for (int i = 0; i < NV; i++) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += PM_WG)
-      acc += __builtin_nontemporal_load(&partials[(size_t)b * NS + i]);
-    acc = pm_wave_sum(acc);
-    __syncthreads();
-    if (lane == 0) sh[i][wave] = acc;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < NV; i++) {
-    double acc = sh[i][0];
-    for (int w = 1; w < PM_WAVES; w++) acc += sh[i][w];
-    out[i] = acc;
-  }
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
 }
 
 inline int pm_grid_for(int64_t n) {

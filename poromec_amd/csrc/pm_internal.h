@@ -156,6 +156,23 @@ struct pm_ctx {
   PmScalars *d_scal = nullptr;
   PmScalars *h_scal = nullptr; // pinned mirror
   int red_blocks = 1024;
+  // live SpMV timing (measurement hook): an event pair around every SpMV a
+  // single-system solver launches, harvested after each host sync; pairs
+  // tagged with an iteration past convergence (launched but skipped by the
+  // kernels' done check) are dropped
+  int time_spmv = 0;
+  int cur_iter = 0;                 // iteration tag of the next launches
+  std::vector<hipEvent_t> tev;      // 2 per pending pair
+  std::vector<int> tev_kind, tev_iter;
+  int tev_used = 0;
+  double spmv_ms[2] = {0.0, 0.0};   // FP64, FP32 values
+  long long spmv_n[2] = {0, 0};
+  // SpMV shape (index into the tuner's table) per value precision
+  int spmv_cfg[2] = {2, 2};
+  int spmv_tuned[2] = {0, 0};
+  // reference summation order: every reduction a sequential left-to-right
+  // sum, the SpMV in the order of matvec_csrc_sym_pm (slow; for parity)
+  int ordered = 0;
 
   // batching form
   PmBatch *batch = nullptr;
@@ -192,3 +209,5 @@ int pm_batch_true_res(pm_ctx *c, const double *b, const double *x, double *res,
 int pm_adjoint_dev(pm_ctx *c, int nsteps, double dt, const pm_solver_cfg *cfg, int nsens,
                    const int *sensor_dof, const double *target, int ckpt_every, double *J,
                    double *grad_logE, double *grad_logk);
+void pm_spmv_harvest(pm_ctx *c, int done, int iter);
+int pm_batch_spmv_dev(pm_ctx *c, const double *x, double *y, int use32);

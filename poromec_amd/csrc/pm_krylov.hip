@@ -13,54 +13,43 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "pm_device.h"
 #include "pm_functors.h"
+#include "pm_spmv.h"
 #include <vector>
 
 #define PM_SPMV_MAX_GRID 8192
 
 // ---------------------------------------------------------------------------
-// SpMV over node blocks: one wavefront per block, lanes stride the shared
-// column list, every lane carries the (<= 4) rows of the block.
+// SpMV over node blocks (shape (LANES, UNR) from the tuner, pm_spmv.h);
+// optional fused Jacobi scaling and dots y.w, y.y into workgroup partials.
 // ---------------------------------------------------------------------------
-template <typename VT, int NV, bool SCALE_M, class Epi>
+template <typename VT, int LANES, int UNR, int NV, bool SCALE_M>
 __global__ __launch_bounds__(PM_WG) void k_spmv(
     int nrn, const int *__restrict__ fr, const int *__restrict__ nr,
     const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
     const int *__restrict__ ncol, const VT *__restrict__ val, const double *__restrict__ x,
     double *__restrict__ y, const double *__restrict__ w, const double *__restrict__ m,
-    double *partials, unsigned int *counter, PmScalars *S, Epi epi) {
+    double *partials, const PmScalars *S) {
   if (S != nullptr && S->done) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int GROUPS = PM_WG / LANES;
+  const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
   const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
   int64_t k0, k1;
   pm_chunk(nrn, lb, gridDim.x, k0, k1);
   double d0 = 0.0, d1 = 0.0;
-  for (int64_t k = k0 + wave; k < k1; k += PM_WAVES) {
+  for (int64_t k = k0 + grp; k < k1; k += GROUPS) {
     const int64_t cp = colptr[k];
     const int c = (int)(colptr[k + 1] - cp);
-    const int64_t vp = valptr[k];
     const int n = nr[k];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (int j = lane; j < c; j += 64) {
-      const int col = __builtin_nontemporal_load(ncol + cp + j);
-      const double xv = x[col];
-      const VT *v = val + vp + j;
-      a0 += (double)__builtin_nontemporal_load(v) * xv;
-      if (n > 1) a1 += (double)__builtin_nontemporal_load(v + c) * xv;
-      if (n > 2) a2 += (double)__builtin_nontemporal_load(v + 2 * c) * xv;
-      if (n > 3) a3 += (double)__builtin_nontemporal_load(v + 3 * c) * xv;
-    }
-    a0 = pm_wave_sum(a0);
-    if (n > 1) a1 = pm_wave_sum(a1);
-    if (n > 2) a2 = pm_wave_sum(a2);
-    if (n > 3) a3 = pm_wave_sum(a3);
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    pm_spmv_block<VT, LANES, UNR>(lane, cp, c, valptr[k], n, ncol, val, x, a);
     if (lane == 0) {
       const int f = fr[k];
-      double r[4] = {a0, a1, a2, a3};
       for (int q = 0; q < n; q++) {
-        double yq = r[q];
+        double yq = a[q];
         if (SCALE_M) yq *= m[f + q];
         y[f + q] = yq;
         if (NV >= 1) d0 += yq * w[f + q];
@@ -69,34 +58,204 @@ __global__ __launch_bounds__(PM_WG) void k_spmv(
     }
   }
   if (NV > 0) {
-    double v[2] = {d0, d1}, out[2];
-    if (pm_grid_reduce<NV>(v, partials, counter, out) && threadIdx.x == 0) {
-      for (int i = 0; i < NV; i++) S->red[i] = out[i];
-      epi(S, out);
-    }
+    double v[2] = {d0, d1};
+    pm_block_partials<NV>(v, partials);
   }
 }
 
-static int spmv_grid(int nrn) {
-  int g = (nrn + 3) / 4;
+// Sums the workgroup partials of the kernel before it (fixed order) into
+// S->red and runs the epilogue that computes the next solver scalars.
+template <int NV, class Epi>
+__global__ __launch_bounds__(PM_WG) void k_finish(int nblk, const double *partials, PmScalars *S,
+                                                  Epi epi, int ignore_done) {
+  if (!ignore_done && S->done) return;
+  double out[NV > 0 ? NV : 1];
+  pm_sum_partials<NV>(nblk, partials, out);
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < NV; i++) S->red[i] = out[i];
+    epi(S, out);
+  }
+}
+
+template <int NV, class Epi>
+static void launch_finish(pm_ctx *c, int nblk, PmScalars *S, Epi epi, int ignore_done) {
+  if (NV == 0 && std::is_same<Epi, NoEpi>::value) return;
+  hipLaunchKernelGGL((k_finish<NV, Epi>), dim3(1), dim3(PM_WG), 0, c->stream, nblk,
+                     c->d_partials, S, epi, ignore_done);
+}
+
+static int spmv_grid(int nrn, int lanes) {
+  const int groups = PM_WG / lanes;
+  int g = (nrn + groups - 1) / groups;
   if (g < 8) g = 8;
   if (g > PM_SPMV_MAX_GRID) g = PM_SPMV_MAX_GRID;
   return (g + 7) & ~7;
 }
 
+template <typename VT, int NV, bool SCALE_M>
+static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, double *y,
+                             const double *w, const double *m, const PmScalars *S) {
+  const PmDevMatrix &A = c->A;
+  const int g = spmv_grid(A.nrn, pm_spmv_lanes[cfg]);
+#define PM_SPMV_CASE(id, L, U)                                                                  \
+  case id:                                                                                      \
+    hipLaunchKernelGGL((k_spmv<VT, L, U, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream,   \
+                       A.nrn, A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w, m,          \
+                       c->d_partials, S);                                                       \
+    break;
+  switch (cfg) {
+    PM_SPMV_CASE(0, 64, 1)
+    PM_SPMV_CASE(1, 64, 2)
+    PM_SPMV_CASE(2, 64, 4)
+    PM_SPMV_CASE(3, 32, 2)
+    PM_SPMV_CASE(4, 32, 4)
+    default: PM_SPMV_CASE(5, 16, 4)
+  }
+#undef PM_SPMV_CASE
+  return g;
+}
+
+
+// ---------------------------------------------------------------------------
+// generic fused vector kernel: body(i, acc) for each row, NV reductions
+// ---------------------------------------------------------------------------
+// With seq != 0 (reference-order mode, one workgroup) thread 0 alone walks
+// the rows in order, so every sum is the sequential left-to-right sum of the
+// reference's loops (the other lanes contribute exact zeros).
+template <int NV, class Body>
+__global__ __launch_bounds__(PM_WG) void k_vec(int64_t n, const PmScalars *S, double *partials,
+                                               Body body, int ignore_done, int seq) {
+  if (!ignore_done && S->done) return;
+  const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
+  int64_t a, b;
+  pm_chunk(n, lb, gridDim.x, a, b);
+  double acc[NV > 0 ? NV : 1];
+  for (int i = 0; i < (NV > 0 ? NV : 1); i++) acc[i] = 0.0;
+  const PmScalars &s = *S;
+  if (seq) {
+    if (threadIdx.x == 0)
+      for (int64_t i = a; i < b; i++) body(i, s, acc);
+  } else {
+    for (int64_t i = a + threadIdx.x; i < b; i += PM_WG) body(i, s, acc);
+  }
+  pm_block_partials<NV>(acc, partials);
+}
+
+template <int NV, class Body, class Epi>
+static void launch_vec(pm_ctx *c, int64_t n, PmScalars *S, Body body, Epi epi,
+                       int ignore_done = 0) {
+  const int g = c->ordered ? 1 : pm_grid_for(n);
+  hipLaunchKernelGGL((k_vec<NV, Body>), dim3(g), dim3(PM_WG), 0, c->stream, n, S, c->d_partials,
+                     body, ignore_done, c->ordered);
+  launch_finish<NV>(c, g, S, epi, ignore_done);
+}
+
+// ---------------------------------------------------------------------------
+// SpMV in the reference's summation order (matvec_csrc_sym_pm, Matvec.f:
+// 1504-1595, with the lower CSRC): row i is ad(i) x(i), then its lower
+// columns in ascending order, then the transposed terms of the rows below
+// it in ascending order -- i.e. diagonal first, then the stored row left to
+// right skipping the diagonal.  One thread per row.
+// ---------------------------------------------------------------------------
+template <typename VT, bool SCALE_M>
+__global__ __launch_bounds__(PM_WG) void k_spmv_ref(
+    int neq, const int *__restrict__ eq_block, const int *__restrict__ fr,
+    const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
+    const int *__restrict__ ncol, const VT *__restrict__ val, const double *__restrict__ x,
+    double *__restrict__ y, const double *__restrict__ m, const PmScalars *S) {
+  if (S != nullptr && S->done) return;
+  const int row = blockIdx.x * PM_WG + threadIdx.x;
+  if (row >= neq) return;
+  const int k = eq_block[row];
+  const int64_t cp = colptr[k];
+  const int c = (int)(colptr[k + 1] - cp);
+  const VT *v = val + valptr[k] + (int64_t)(row - fr[k]) * c;
+  const int *cols = ncol + cp;
+  int jd = 0;
+  while (jd < c && cols[jd] != row) jd++;
+  double t = jd < c ? (double)v[jd] * x[row] : 0.0;
+  for (int j = 0; j < c; j++)
+    if (j != jd) t = t + (double)v[j] * x[cols[j]];
+  if (SCALE_M) t = t * m[row];
+  y[row] = t;
+}
+
+template <int NV>
+struct SpmvDots {  // the dots fused into the fast SpMV: y.w and y.y
+  const double *y, *w;
+  __device__ void operator()(int64_t i, const PmScalars &, double *acc) const {
+    const double yi = y[i];
+    if (NV >= 1) acc[0] += yi * w[i];
+    if (NV >= 2) acc[1] += yi * yi;
+  }
+};
+
+static void spmv_time_begin(pm_ctx *c) {
+  if (!c->time_spmv) return;
+  if ((size_t)(2 * c->tev_used + 2) > c->tev.size()) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      c->time_spmv = 0;  // timing is best effort; the solve goes on
+      return;
+    }
+    c->tev.push_back(a);
+    c->tev.push_back(b);
+    c->tev_kind.push_back(0);
+    c->tev_iter.push_back(0);
+  }
+  (void)hipEventRecord(c->tev[2 * c->tev_used], c->stream);
+}
+
+static void spmv_time_end(pm_ctx *c, int use32) {
+  if (!c->time_spmv || (size_t)(2 * c->tev_used + 2) > c->tev.size()) return;
+  (void)hipEventRecord(c->tev[2 * c->tev_used + 1], c->stream);
+  c->tev_kind[c->tev_used] = use32 ? 1 : 0;
+  c->tev_iter[c->tev_used] = c->cur_iter;
+  c->tev_used++;
+}
+
+// after a host sync: add the durations of the recorded pairs; with done != 0
+// pairs tagged past `iter` ran on a finished solve and are dropped
+void pm_spmv_harvest(pm_ctx *c, int done, int iter) {
+  for (int i = 0; i < c->tev_used; i++) {
+    if (done && c->tev_iter[i] > iter) continue;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->tev[2 * i], c->tev[2 * i + 1]) == hipSuccess) {
+      c->spmv_ms[c->tev_kind[i]] += ms;
+      c->spmv_n[c->tev_kind[i]]++;
+    }
+  }
+  c->tev_used = 0;
+}
+
+template <int NV, bool SCALE_M, class Epi>
+static void launch_spmv_raw(pm_ctx *c, int use32, const double *x, double *y, const double *w,
+                            const double *m, PmScalars *S, Epi epi) {
+  const PmDevMatrix &A = c->A;
+  if (c->ordered) {
+    const int g = (A.neq + PM_WG - 1) / PM_WG;
+    if (use32)
+      hipLaunchKernelGGL((k_spmv_ref<float, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream,
+                         A.neq, A.eq_block, A.fr, A.colptr, A.valptr, A.ncol, A.val32, x, y, m,
+                         S);
+    else
+      hipLaunchKernelGGL((k_spmv_ref<double, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream,
+                         A.neq, A.eq_block, A.fr, A.colptr, A.valptr, A.ncol, A.val, x, y, m, S);
+    if (NV > 0) launch_vec<NV>(c, A.neq, S, SpmvDots<NV>{y, w}, epi, 0);
+    return;
+  }
+  const int cfg = c->spmv_cfg[use32 ? 1 : 0];
+  const int g = use32 ? launch_spmv_shape<float, NV, SCALE_M>(c, cfg, A.val32, x, y, w, m, S)
+                      : launch_spmv_shape<double, NV, SCALE_M>(c, cfg, A.val, x, y, w, m, S);
+  if (NV > 0) launch_finish<NV>(c, g, S, epi, 0);
+}
+
 template <int NV, bool SCALE_M, class Epi>
 static void launch_spmv(pm_ctx *c, int use32, const double *x, double *y, const double *w,
                         const double *m, PmScalars *S, Epi epi) {
-  const PmDevMatrix &A = c->A;
-  int g = spmv_grid(A.nrn);
-  if (use32)
-    hipLaunchKernelGGL((k_spmv<float, NV, SCALE_M, Epi>), dim3(g), dim3(PM_WG), 0, c->stream,
-                       A.nrn, A.fr, A.nr, A.colptr, A.valptr, A.ncol, A.val32, x, y, w, m,
-                       c->d_partials, c->d_counter, S, epi);
-  else
-    hipLaunchKernelGGL((k_spmv<double, NV, SCALE_M, Epi>), dim3(g), dim3(PM_WG), 0, c->stream,
-                       A.nrn, A.fr, A.nr, A.colptr, A.valptr, A.ncol, A.val, x, y, w, m,
-                       c->d_partials, c->d_counter, S, epi);
+  spmv_time_begin(c);
+  launch_spmv_raw<NV, SCALE_M>(c, use32, x, y, w, m, S, epi);
+  spmv_time_end(c, use32);
 }
 
 int pm_spmv_dev(pm_ctx *c, const double *x, double *y, int use32) {
@@ -105,43 +264,13 @@ int pm_spmv_dev(pm_ctx *c, const double *x, double *y, int use32) {
   return PM_OK;
 }
 
-// ---------------------------------------------------------------------------
-// generic fused vector kernel: body(i, acc) for each row, NV reductions
-// ---------------------------------------------------------------------------
-template <int NV, class Body, class Epi>
-__global__ __launch_bounds__(PM_WG) void k_vec(int64_t n, PmScalars *S, double *partials,
-                                               unsigned int *counter, Body body, Epi epi,
-                                               int ignore_done) {
-  if (!ignore_done && S->done) return;
-  const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
-  int64_t a, b;
-  pm_chunk(n, lb, gridDim.x, a, b);
-  double acc[NV > 0 ? NV : 1];
-  for (int i = 0; i < (NV > 0 ? NV : 1); i++) acc[i] = 0.0;
-  const PmScalars &s = *S;
-  for (int64_t i = a + threadIdx.x; i < b; i += PM_WG) body(i, s, acc);
-  double out[NV > 0 ? NV : 1];
-  // NV == 0 still elects the last workgroup so that the epilogue runs once
-  if (pm_grid_reduce<NV>(acc, partials, counter, out) && threadIdx.x == 0) {
-    for (int i = 0; i < NV; i++) S->red[i] = out[i];
-    epi(S, out);
-  }
-}
-
-template <int NV, class Body, class Epi>
-static void launch_vec(pm_ctx *c, int64_t n, PmScalars *S, Body body, Epi epi,
-                       int ignore_done = 0) {
-  int g = pm_grid_for(n);
-  hipLaunchKernelGGL((k_vec<NV, Body, Epi>), dim3(g), dim3(PM_WG), 0, c->stream, n, S,
-                     c->d_partials, c->d_counter, body, epi, ignore_done);
-}
-
 static int poll_done(pm_ctx *c, int *done, int *iter) {
   PM_CHECK_HIP(hipMemcpyAsync(c->h_scal, c->d_scal, sizeof(PmScalars), hipMemcpyDeviceToHost,
                               c->stream));
   PM_CHECK_HIP(hipStreamSynchronize(c->stream));
   *done = c->h_scal->done;
   *iter = c->h_scal->iter;
+  if (c->tev_used) pm_spmv_harvest(c, *done, *iter);
   return PM_OK;
 }
 
@@ -568,6 +697,7 @@ static int run_pcg(pm_ctx *c, int n, const double *b, double *x, const double *m
   while (launched < maxit) {
     int chunk = std::min(next_chunk(launched), maxit - launched);
     for (int it = 0; it < chunk; it++) {
+      c->cur_iter = launched + it + 1;
       launch_spmv<1, false>(c, use32, p, z, p, nullptr, S, PcgAlpha{});
       (use32 ? info->spmv32 : info->spmv64)++;
       launch_vec<1>(c, n, S, PcgUpd{x, r, z, p, m}, PcgUpdEpi{});
@@ -596,6 +726,7 @@ static int run_sqmr(pm_ctx *c, int n, const double *b, double *x, const double *
   while (launched < maxit) {
     int chunk = std::min(next_chunk(launched), maxit - launched);
     for (int it = 0; it < chunk; it++) {
+      c->cur_iter = launched + it + 1;
       launch_spmv<1, false>(c, use32, q, t, q, nullptr, S, SqmrSigma{});
       (use32 ? info->spmv32 : info->spmv64)++;
       launch_vec<1>(c, n, S, SqmrR{r, t}, SqmrREpi{});
@@ -629,6 +760,7 @@ static int run_bicgstab(pm_ctx *c, int n, const double *b, double *x, const doub
   while (launched < maxit) {
     int chunk = std::min(next_chunk(launched), maxit - launched);
     for (int it = 0; it < chunk; it++) {
+      c->cur_iter = launched + it + 1;
       launch_spmv<1, false>(c, use32, z, v, r0, nullptr, S, BcgAlpha{});
       launch_vec<1>(c, n, S, BcgS{x, r, z, v, m}, BcgSEpi{});
       launch_spmv<2, false>(c, use32, z, t, r, nullptr, S, BcgW{});
@@ -657,7 +789,7 @@ static int run_minres(pm_ctx *c, int n, const double *b, double *x, const double
   PM_TRY(reset_scalars(c));
   hipMemsetAsync(x, 0, sizeof(double) * n, c->stream);
   launch_vec<1>(c, n, S, MrConv{b, m}, MrConvEpi{tol});
-  int done = 0, iter = 0;
+  int done = 0, iter = 0, total = 0;
   for (int cycle = 0; cycle <= nrestart && !done; cycle++) {
     double *v0 = buf[0], *v = buf[1], *p = buf[2], *z0 = buf[3], *z = buf[4], *w00 = buf[5],
            *w0 = buf[6];
@@ -667,6 +799,7 @@ static int run_minres(pm_ctx *c, int n, const double *b, double *x, const double
     while (launched < maxit) {
       int chunk = std::min(next_chunk(launched), maxit - launched);
       for (int it = 0; it < chunk; it++) {
+        c->cur_iter = total + launched + it + 1;
         launch_spmv<1, false>(c, use32, z0, p, z0, nullptr, S, MrAlpha{});
         (use32 ? info->spmv32 : info->spmv64)++;
         launch_vec<1>(c, n, S, MrP{p, z, v, v0, m}, MrQr{});
@@ -689,6 +822,7 @@ static int run_minres(pm_ctx *c, int n, const double *b, double *x, const double
       PM_TRY(poll_done(c, &done, &iter));
       if (done) break;
     }
+    total += launched;
   }
   if (done < 0) return done;
   if (!done) return PM_ERR_MAXIT;
@@ -974,6 +1108,7 @@ int pm_solve_dev(pm_ctx *c, const pm_solver_cfg *cfg, const double *b, double *x
       info->outer++;
       launch_vec<0>(c, n, c->d_scal, AxpyBody{x, dx}, NoEpi{}, 1);
       // FP64 residual of the outer iterate
+      c->cur_iter = 0;
       launch_spmv<0, false>(c, 0, x, res, nullptr, nullptr, nullptr, NoEpi{});
       info->spmv64++;
       PM_TRY(reset_scalars(c));
@@ -990,11 +1125,13 @@ int pm_solve_dev(pm_ctx *c, const pm_solver_cfg *cfg, const double *b, double *x
     return PM_ERR_ARG;
   }
   // true FP64 residual of the result
+  c->cur_iter = 0;
   launch_spmv<0, false>(c, 0, x, res, nullptr, nullptr, nullptr, NoEpi{});
   PM_TRY(reset_scalars(c));
   launch_vec<2>(c, n, c->d_scal, TrueRes{b, res}, RedEpi{});
   double r2 = 0.0, b2 = 0.0;
   PM_TRY(read_red(c, &r2, &b2));
+  if (c->tev_used) pm_spmv_harvest(c, 0, 0);
   info->true_rel = b2 > 0 ? sqrt(r2 / b2) : 0.0;
   PM_CHECK_HIP(hipEventRecord(c->ev1, c->stream));
   PM_CHECK_HIP(hipEventSynchronize(c->ev1));

@@ -130,17 +130,23 @@ __global__ void k_update_res(MarchArgs A) {
 
 // ||R_u||^2 and ||R_p||^2 over the free equations (pressure flag per eq)
 __global__ __launch_bounds__(PM_WG) void k_resnorm(int n, const double *b, const unsigned char *isp,
-                                                   double *partials, unsigned int *counter,
-                                                   double *out) {
+                                                   double *partials) {
   const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
   int64_t a, e;
   pm_chunk(n, lb, gridDim.x, a, e);
-  double v[2] = {0.0, 0.0}, r[2];
+  double v[2] = {0.0, 0.0};
   for (int64_t i = a + threadIdx.x; i < e; i += PM_WG) {
     const double bi = b[i];
     v[isp[i]] += bi * bi;
   }
-  if (pm_grid_reduce<2>(v, partials, counter, r) && threadIdx.x == 0) {
+  pm_block_partials<2>(v, partials);
+}
+
+__global__ __launch_bounds__(PM_WG) void k_resnorm_sum(int nblk, const double *partials,
+                                                       double *out) {
+  double r[2];
+  pm_sum_partials<2>(nblk, partials, r);
+  if (threadIdx.x == 0) {
     out[0] = r[0];
     out[1] = r[1];
   }
@@ -172,18 +178,47 @@ static MarchArgs march_args(pm_ctx *c, double dt) {
   return A;
 }
 
-// residual norms (u, p) on the host
-static int residual_norms(pm_ctx *c, const double *b, double *ru2, double *rp2) {
+// The same three sums in the reference's order (cal_residuo_pm,
+// Residuo.f:39-136): b.b over the equations, then ||R_u||^2 and ||R_p||^2
+// node by node.  One thread; reference-order mode only.
+__global__ void k_resnorm_ref(int nnode, int neq, const int *eq, const double *b, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double all = 0.0, ru = 0.0, rp = 0.0;
+  for (int i = 0; i < neq; i++) all = all + b[i] * b[i];
+  for (int i = 0; i < nnode; i++)
+    for (int j = 0; j < 3; j++) {
+      const int q = eq[(size_t)i * 4 + j];
+      if (q >= 0) ru = ru + b[q] * b[q];
+    }
+  for (int i = 0; i < nnode; i++) {
+    const int q = eq[(size_t)i * 4 + 3];
+    if (q >= 0) rp = rp + b[q] * b[q];
+  }
+  out[0] = ru;
+  out[1] = rp;
+  out[2] = all;
+}
+
+// residual norms on the host: ||R_u||^2, ||R_p||^2 and ||R||^2
+static int residual_norms(pm_ctx *c, const double *b, double *ru2, double *rp2, double *r2) {
   int n = c->mesh.neq;
-  int g = pm_grid_for(n);
   double *out = c->d_wk[11];
-  hipLaunchKernelGGL(k_resnorm, dim3(g), dim3(PM_WG), 0, c->stream, n, b, c->d_isp,
-                     c->d_partials, c->d_counter, out);
-  double h[2];
-  PM_CHECK_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  if (c->ordered) {
+    hipLaunchKernelGGL(k_resnorm_ref, dim3(1), dim3(64), 0, c->stream, c->mesh.nnode, n, c->d_eq,
+                       b, out);
+  } else {
+    int g = pm_grid_for(n);
+    hipLaunchKernelGGL(k_resnorm, dim3(g), dim3(PM_WG), 0, c->stream, n, b, c->d_isp,
+                       c->d_partials);
+    hipLaunchKernelGGL(k_resnorm_sum, dim3(1), dim3(PM_WG), 0, c->stream, g, c->d_partials, out);
+  }
+  double h[3];
+  PM_CHECK_HIP(hipMemcpyAsync(h, out, sizeof(double) * (c->ordered ? 3 : 2),
+                              hipMemcpyDeviceToHost, c->stream));
   PM_CHECK_HIP(hipStreamSynchronize(c->stream));
   *ru2 = h[0];
   *rp2 = h[1];
+  *r2 = c->ordered ? h[2] : h[0] + h[1];
   return PM_OK;
 }
 
@@ -278,11 +313,11 @@ int pm_step_dev(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_info *in
         lhs = 0;
         continue;
       }
-      double ru2, rp2;
-      if ((rc = residual_norms(c, c->d_b, &ru2, &rp2))) break;
+      double ru2, rp2, r2;
+      if ((rc = residual_norms(c, c->d_b, &ru2, &rp2, &r2))) break;
       int istop = 0;
       if (cfg->stop_crit == 1) {
-        const double res = sqrt(ru2 + rp2);
+        const double res = sqrt(r2);
         if (i == 1) s0 = res;
         else if (i == 2) s0 = fmax(s0, res);
         if (res / s0 < cfg->nltol) istop = 2;

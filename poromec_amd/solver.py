@@ -22,6 +22,7 @@ from .problem import NDF, NEN, Deck
 
 SOLVERS = {"pcg": 1, "gmres": 2, "bicgstab": 4, "minres": 7, "sqmr": 11}
 SOLVER_NAMES = {v: k for k, v in SOLVERS.items()}
+SPMV_SHAPES = 6  # PM_SPMV_SHAPES: (lanes per node block, columns in flight per lane) choices
 
 
 @dataclass
@@ -170,6 +171,11 @@ class MefSolver:
                     used=list(info.used[:n]), true_rel=list(info.true_rel[:n]),
                     ms_total=info.ms_total, ms_solve=info.ms_solve, ms_element=info.ms_element)
 
+    def set_ordered(self, on: bool = True):
+        """Sum every reduction and SpMV row in the reference's sequential
+        order (slow; for bit-level parity checks)."""
+        L.check(self.lib.pm_set_ordered(self.ctx, int(on)), "pm_set_ordered")
+
     # -- adjoint form -----------------------------------------------------------
     def adjoint(self, nsteps: int, sensors, target, cfg: SolverConfig | None = None,
                 dt: float | None = None, ckpt_every: int = 8):
@@ -219,6 +225,25 @@ class MefSolver:
         L.check(self.lib.pm_bench_spmv(self.ctx, int(use32), reps, C.byref(ms)), "pm_bench_spmv")
         return ms.value
 
+    def tune_spmv(self, use32: bool, reps: int = 10):
+        """Time every SpMV shape on the current matrix and keep the fastest;
+        returns (best shape, ms per launch of each shape)."""
+        ms = np.zeros(SPMV_SHAPES)
+        best = C.c_int()
+        L.check(self.lib.pm_tune_spmv(self.ctx, int(use32), int(reps), C.byref(best), L.ptr(ms)),
+                "pm_tune_spmv")
+        return best.value, ms
+
+    def set_spmv_shape(self, use32: bool, shape: int):
+        L.check(self.lib.pm_set_spmv_shape(self.ctx, int(use32), int(shape)), "pm_set_spmv_shape")
+
+    def spmv_timing(self, on: bool) -> dict:
+        """Switch live SpMV timing on or off; returns what was accumulated
+        since the previous call (ms and launches per value precision)."""
+        st = np.zeros(4)
+        L.check(self.lib.pm_spmv_timing(self.ctx, int(on), L.ptr(st)), "pm_spmv_timing")
+        return dict(fp64_ms=st[0], fp64_n=int(st[1]), fp32_ms=st[2], fp32_n=int(st[3]))
+
     def bench_copy(self, nbytes: int, reps: int = 20) -> float:
         ms = C.c_double()
         L.check(self.lib.pm_bench_copy(self.ctx, int(nbytes), reps, C.byref(ms)), "pm_bench_copy")
@@ -248,6 +273,10 @@ class CsrcSolver:
         self.neq = len(self.ia) - 1
         L.check(self.lib.pm_struct_upload(self.ctx, self.neq, int(self.ia[-1] - 1),
                                           L.ptr(self.ia), L.ptr(self.ja)), "pm_struct_upload")
+
+    def set_ordered(self, on: bool = True):
+        """Reference summation order (see MefSolver.set_ordered)."""
+        L.check(self.lib.pm_set_ordered(self.ctx, int(on)), "pm_set_ordered")
 
     def set_values(self, ad, al):
         L.check(self.lib.pm_values_upload(self.ctx, L.ptr(L.as_f64(ad)), L.ptr(L.as_f64(al))),
