@@ -1,11 +1,13 @@
 #!/usr/bin/env python3
 """Benchmark of the mef Biot u-p path on one MI355X.
 
-Workload: a box of n^3 hex8 cells (default 128^3) promoted to the reference's
-hexa20/8 element, with a seeded log-normal permeability field (sigma 1,
-correlation length 8 cells), E ~ U[5, 50] MPa and nu ~ U[0.25, 0.35] per
-element, a seeded 10 % patch of 1 kPa top load, dt = 0.1 s, SQMR with Jacobi
-to 1e-10.  A "step" is one backward-Euler time step of the solv macro.
+Workload: a box of n^3 hex8 cells promoted to the reference's hexa20/8
+element, with a seeded log-normal permeability field (sigma 1, correlation
+length 8 cells), E ~ U[5, 50] MPa and nu ~ U[0.25, 0.35] per element, a seeded
+10 % patch of 1 kPa top load, dt = 0.1 s, SQMR with Jacobi to 1e-10.  A "step"
+is one backward-Euler time step of the solv macro.  The default n = 87 gives
+8.67M equations, the 8.6M-dof size the metric is quoted at (with the
+reference's element a 128^3-cell box has 27.6M equations; `--n 128` runs it).
 
 The FP64 form and the FP32+IR form (FP32 matrix values and Jacobi inside an
 FP64 iterative-refinement loop) each run W untimed steps, then exactly K
@@ -120,8 +122,8 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=128, help="box edge in cells")
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--n", type=int, default=87, help="box edge in cells")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--cpu-sample-n", type=int, default=10)

@@ -87,13 +87,14 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, const int *__rest
   int64_t k0, k1;
   pm_chunk(nblk, blockIdx.x - gb, gc, k0, k1);
   double d0 = 0.0, d1 = 0.0;
+  PmBlockMeta nx = pm_block_meta(nb0 + k0 + grp, nb0 + k1, colptr, valptr, nr);
   for (int64_t kk = k0 + grp; kk < k1; kk += GROUPS) {
     const int64_t k = nb0 + kk;
-    const int64_t cp = colptr[k];
-    const int c = (int)(colptr[k + 1] - cp);
-    const int n = nr[k];
+    const PmBlockMeta cur = nx;
+    nx = pm_block_meta(k + GROUPS, nb0 + k1, colptr, valptr, nr);
+    const int n = cur.n;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    pm_spmv_block<VT, LANES, UNR>(lane, cp, c, valptr[k], n, ncol, val, x, a);
+    pm_spmv_block<VT, LANES, UNR>(lane, cur.cp, cur.c, cur.vp, n, ncol, val, x, a);
     if (lane == 0) {
       const int f = fr[k];
       for (int q = 0; q < n; q++) {

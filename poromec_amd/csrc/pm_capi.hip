@@ -54,6 +54,15 @@ static void free_matrix(PmDevMatrix &A) {
 
 pm_ctx::~pm_ctx() {
   for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+  FREE(d_omptr);
+  FREE(d_omsrc);
+  FREE(d_orptr);
+  FREE(d_orsrc);
+  FREE(d_opptr);
+  FREE(d_opsrc);
+  FREE(d_ke);
+  FREE(d_pe);
+  FREE(d_pv);
   if (batch) pm_batch_free(batch);
   FREE(d_eq_box);
   free_matrix(A);

@@ -18,6 +18,8 @@
 
 #include "pm_device.h"
 
+#include <algorithm>
+
 #include "pm_elemlib.h"
 
 struct ElemArgs {
@@ -43,6 +45,9 @@ struct ElemArgs {
   int lhs, rhs;
   const int *elems;    // elements of this colour
   int nelem;
+  // reference summation order: element arrays are stored, not scattered
+  double *ke;          // (numel, 68, 68) element matrices, or null
+  double *pe;          // (numel, 68) element vectors, or null
 };
 
 #define GP_CHUNK 16
@@ -192,10 +197,19 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw2(ElemArgs A) {
   if (A.rhs && tid < PM_NST) {
     double p = 0.0;
     for (int j = 0; j < PM_NST; j++) p = p + s_K[tid][j] * s_u[j];
-    const int k = s_ld[tid];
-    if (k >= 0) A.b[k] = A.b[k] - p;
+    if (A.pe) {
+      A.pe[(size_t)el * PM_NST + tid] = p;
+    } else {
+      const int k = s_ld[tid];
+      if (k >= 0) A.b[k] = A.b[k] - p;
+    }
   }
   if (!A.lhs) return;
+  if (A.ke) {  // element order assembly: keep K_e for the gather
+    double *ke = A.ke + (size_t)el * PM_NST * PM_NST;
+    for (int q = tid; q < PM_NST * PM_NST; q += PM_WG) ke[q] = s_K[q / PM_NST][q % PM_NST];
+    return;
+  }
   // scatter node-pair blocks: rows of node a, columns of node b
   for (int pr = tid; pr < 400; pr += PM_WG) {
     const int a = pr / 20, b = pr % 20;
@@ -391,7 +405,10 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw4(ElemArgs A) {
   __shared__ double s_dp[8];
   __shared__ int s_ld[PM_NST];
   __shared__ double s_mat[PM_PROP];
-  __shared__ double s_c[64][PM_NST + 1];  // contribution of each Gauss point
+  // per Gauss point, the two terms the reference adds and subtracts in turn:
+  // stress (flux) term, then gravity (permeation) term
+  __shared__ double s_c[64][PM_NST + 1];
+  __shared__ double s_c2[64][PM_NST + 1];
   __shared__ double s_p[PM_NST];
 
   const int tid = threadIdx.x;
@@ -456,9 +473,12 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw4(ElemArgs A) {
     const int k0 = 7 * sub, k1 = min(20, k0 + 7);
     for (int k = k0; k < k1; k++) {
       const double wt3 = hv[k] * wt2;
-      s_c[g][3 * k] = w * (hgx[k] * tx[0] + hgy[k] * tx[3] + hgz[k] * tx[5]) - wt3 * A.grav[0];
-      s_c[g][3 * k + 1] = w * (hgx[k] * tx[3] + hgy[k] * tx[1] + hgz[k] * tx[4]) - wt3 * A.grav[1];
-      s_c[g][3 * k + 2] = w * (hgx[k] * tx[5] + hgy[k] * tx[4] + hgz[k] * tx[2]) - wt3 * A.grav[2];
+      s_c[g][3 * k] = w * (hgx[k] * tx[0] + hgy[k] * tx[3] + hgz[k] * tx[5]);
+      s_c[g][3 * k + 1] = w * (hgx[k] * tx[3] + hgy[k] * tx[1] + hgz[k] * tx[4]);
+      s_c[g][3 * k + 2] = w * (hgx[k] * tx[5] + hgy[k] * tx[4] + hgz[k] * tx[2]);
+      s_c2[g][3 * k] = wt3 * A.grav[0];
+      s_c2[g][3 * k + 1] = wt3 * A.grav[1];
+      s_c2[g][3 * k + 2] = wt3 * A.grav[2];
     }
   } else {
     double gx[8], gy[8], gz[8];
@@ -472,13 +492,17 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw4(ElemArgs A) {
       double tmp = 0.0;
       for (int j = 0; j < 8; j++)
         tmp = tmp + wt2 * s_u[60 + j] * (gx[i] * gx[j] + gy[i] * gy[j] + gz[i] * gz[j]);
-      s_c[g][60 + i] = wt1 * (gx[i] * A.grav[0] + gy[i] * A.grav[1] + gz[i] * A.grav[2]) - tmp;
+      s_c[g][60 + i] = wt1 * (gx[i] * A.grav[0] + gy[i] * A.grav[1] + gz[i] * A.grav[2]);
+      s_c2[g][60 + i] = tmp;
     }
   }
   __syncthreads();
   if (tid < PM_NST) {
     double p = 0.0;
-    for (int q = 0; q < 64; q++) p = p + s_c[q][tid];  // (lz, ly, lx) order
+    for (int q = 0; q < 64; q++) {  // (lz, ly, lx) order, two updates per point
+      p = p + s_c[q][tid];
+      p = p - s_c2[q][tid];
+    }
     s_p[tid] = p;
   }
   __syncthreads();
@@ -490,8 +514,12 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw4(ElemArgs A) {
   }
   __syncthreads();
   if (A.rhs && tid < PM_NST) {
-    const int k = s_ld[tid];
-    if (k >= 0) A.b[k] = A.b[k] - s_p[tid];
+    if (A.pe) {
+      A.pe[(size_t)el * PM_NST + tid] = s_p[tid];
+    } else {
+      const int k = s_ld[tid];
+      if (k >= 0) A.b[k] = A.b[k] - s_p[tid];
+    }
   }
 }
 
@@ -499,7 +527,7 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw4(ElemArgs A) {
 // isw = 7: nodal porosity change alpha tr(eps) + dp / M at the 8 vertices,
 // summed per vertex (porosity_form, Pform.f:969-1176)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(PM_WG) void k_elmt_isw7(ElemArgs A, double *dporo) {
+__global__ __launch_bounds__(PM_WG) void k_elmt_isw7(ElemArgs A, double *dporo, double *pv) {
   const int tid = threadIdx.x;
   const int le = blockIdx.x * (PM_WG / 8) + (tid >> 3);
   const int v = tid & 7;
@@ -525,8 +553,39 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw7(ElemArgs A, double *dporo) 
     wz = wz + gz * A.u[(size_t)no * 4 + 2];
   }
   const int no = nodes[v];
-  const double pv = e[4] * (ux + vy + wz) + (1.0 / e[3]) * A.u[(size_t)no * 4 + 3];
-  dporo[no] += pv;  // colour order: no two elements of a colour share a vertex
+  const double val = e[4] * (ux + vy + wz) + (1.0 / e[3]) * A.u[(size_t)no * 4 + 3];
+  if (pv) pv[(size_t)el * 8 + v] = val;  // gathered per node in element order
+  else dporo[no] += val;  // colour order: no two elements of a colour share a vertex
+}
+
+// ---------------------------------------------------------------------------
+// element-order gathers (reference summation order)
+// ---------------------------------------------------------------------------
+__global__ void k_gather_vals(int64_t nnz, const int64_t *ptr, const int64_t *src,
+                              const double *ke, double *val) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnz) return;
+  double v = 0.0;
+  for (int64_t i = ptr[t]; i < ptr[t + 1]; i++) v = v + ke[src[i]];
+  val[t] = v;
+}
+
+__global__ void k_gather_rhs(int neq, const int64_t *ptr, const int64_t *src, const double *pe,
+                             double *b) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= neq) return;
+  double v = b[r];
+  for (int64_t i = ptr[r]; i < ptr[r + 1]; i++) v = v - pe[src[i]];
+  b[r] = v;
+}
+
+__global__ void k_gather_poro(int nnode, const int64_t *ptr, const int64_t *src, const double *pv,
+                              double *dporo) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnode) return;
+  double v = 0.0;
+  for (int64_t i = ptr[n]; i < ptr[n + 1]; i++) v = v + pv[src[i]];
+  dporo[n] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -558,7 +617,38 @@ static ElemArgs elem_args(pm_ctx *c, double dt, double t, double *bvec, int lhs,
   A.val = c->A.val;
   A.lhs = lhs;
   A.rhs = rhs;
+  A.ke = nullptr;
+  A.pe = nullptr;
   return A;
+}
+
+// element-order maps and element arrays for the reference summation order
+static int ensure_ordered(pm_ctx *c) {
+  if (c->have_ordered_maps) return PM_OK;
+  const int64_t numel = c->mesh.numel;
+  if (numel > 50000) return PM_ERR_UNSUPPORTED;  // element matrices would exceed ~1.8 GB
+  PmHostPattern p;
+  PM_TRY(pm_host_pattern(c->mesh, p));
+  PmOrderedMaps o;
+  PM_TRY(pm_host_ordered_maps(c->mesh, p, o));
+  auto up = [&](int64_t **d, const std::vector<int64_t> &h) -> int {
+    if (*d) (void)hipFree(*d);
+    *d = nullptr;
+    PM_CHECK_HIP(hipMalloc((void **)d, sizeof(int64_t) * std::max<size_t>(h.size(), 1)));
+    PM_CHECK_HIP(hipMemcpy(*d, h.data(), sizeof(int64_t) * h.size(), hipMemcpyHostToDevice));
+    return PM_OK;
+  };
+  PM_TRY(up(&c->d_omptr, o.mptr));
+  PM_TRY(up(&c->d_omsrc, o.msrc));
+  PM_TRY(up(&c->d_orptr, o.rptr));
+  PM_TRY(up(&c->d_orsrc, o.rsrc));
+  PM_TRY(up(&c->d_opptr, o.pptr));
+  PM_TRY(up(&c->d_opsrc, o.psrc));
+  PM_CHECK_HIP(hipMalloc((void **)&c->d_ke, sizeof(double) * numel * PM_NST * PM_NST));
+  PM_CHECK_HIP(hipMalloc((void **)&c->d_pe, sizeof(double) * numel * PM_NST));
+  PM_CHECK_HIP(hipMalloc((void **)&c->d_pv, sizeof(double) * numel * 8));
+  c->have_ordered_maps = true;
+  return PM_OK;
 }
 
 static int read_elem_err(pm_ctx *c) {
@@ -577,6 +667,11 @@ int pm_pform_dev(pm_ctx *c, int isw, int lhs, int rhs, double dt, double t, doub
     c->A.have32 = false;
   }
   ElemArgs A = elem_args(c, dt, t, bvec, lhs, rhs);
+  if (c->ordered) {
+    PM_TRY(ensure_ordered(c));
+    if (isw == 2 && lhs) A.ke = c->d_ke;
+    if (rhs) A.pe = c->d_pe;
+  }
   const int ncolour = (int)c->colour_ptr.size() - 1;
   for (int k = 0; k < ncolour; k++) {
     A.elems = c->d_colour_elems + c->colour_ptr[k];
@@ -588,6 +683,18 @@ int pm_pform_dev(pm_ctx *c, int isw, int lhs, int rhs, double dt, double t, doub
       hipLaunchKernelGGL(k_elmt_isw4, dim3(A.nelem), dim3(PM_WG), 0, c->stream, A);
     else
       return PM_ERR_ARG;
+  }
+  if (c->ordered) {  // sums in element order, as the reference's assembly
+    if (A.ke) {
+      const int64_t nnz = c->A.nnz;
+      hipLaunchKernelGGL(k_gather_vals, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0,
+                         c->stream, nnz, c->d_omptr, c->d_omsrc, c->d_ke, c->A.val);
+    }
+    if (A.pe) {
+      const int neq = c->mesh.neq;
+      hipLaunchKernelGGL(k_gather_rhs, dim3((neq + 255) / 256), dim3(256), 0, c->stream, neq,
+                         c->d_orptr, c->d_orsrc, c->d_pe, bvec);
+    }
   }
   PM_CHECK_HIP(hipGetLastError());
   int err = read_elem_err(c);
@@ -608,13 +715,23 @@ int pm_porosity_dev(pm_ctx *c, double dt) {
   PM_CHECK_HIP(hipMemsetAsync(c->d_dporo, 0, sizeof(double) * c->mesh.nnode, c->stream));
   PM_CHECK_HIP(hipMemsetAsync(c->d_elem_err, 0, sizeof(int), c->stream));
   ElemArgs A = elem_args(c, dt, c->t, nullptr, 0, 0);
+  double *pv = nullptr;
+  if (c->ordered) {
+    PM_TRY(ensure_ordered(c));
+    pv = c->d_pv;
+  }
   const int ncolour = (int)c->colour_ptr.size() - 1;
   for (int k = 0; k < ncolour; k++) {
     A.elems = c->d_colour_elems + c->colour_ptr[k];
     A.nelem = c->colour_ptr[k + 1] - c->colour_ptr[k];
     if (A.nelem == 0) continue;
     int g = (A.nelem + PM_WG / 8 - 1) / (PM_WG / 8);
-    hipLaunchKernelGGL(k_elmt_isw7, dim3(g), dim3(PM_WG), 0, c->stream, A, c->d_dporo);
+    hipLaunchKernelGGL(k_elmt_isw7, dim3(g), dim3(PM_WG), 0, c->stream, A, c->d_dporo, pv);
+  }
+  if (pv) {
+    const int nn = c->mesh.nnode;
+    hipLaunchKernelGGL(k_gather_poro, dim3((nn + 255) / 256), dim3(256), 0, c->stream, nn,
+                       c->d_opptr, c->d_opsrc, pv, c->d_dporo);
   }
   int nv = c->mesh.nnodev;
   hipLaunchKernelGGL(k_poro_update, dim3((nv + 255) / 256), dim3(256), 0, c->stream, nv, c->d_ic,

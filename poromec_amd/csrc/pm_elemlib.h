@@ -30,45 +30,139 @@ __device__ __forceinline__ double n8(int k, double r, double s, double t) {
   return (1.0 + c_rn[k] * r) * (1.0 + c_sn[k] * s) * (1.0 + c_tn[k] * t) * 0.125;
 }
 
-// natural derivatives (and value) of serendipity hexa20 node k (0..19)
+// value and natural derivatives of serendipity hexa20 node k (0..19): the
+// reference's per-node expressions of sfhexa20 (Elmlib.f:1590-1807) term
+// for term, so that with contraction off they round as the reference does.
+// Nodes 8..11 are the mid-edges of the t = +1 face, 12..15 of t = -1, and
+// 16..19 the vertical edges under vertices 0..3.
 __device__ void dn20(int k, double r, double s, double t, double &h, double &dr, double &ds,
                      double &dt) {
-  if (k < 8) {
-    const double a = c_rn[k], b = c_sn[k], c = c_tn[k];
-    const double pr = 1.0 + a * r, ps = 1.0 + b * s, pt = 1.0 + c * t;
-    const double g = a * r + b * s + c * t - 2.0;
-    h = pr * ps * pt * g * 0.125;
-    dr = a * ps * pt * (g + pr) * 0.125;
-    ds = b * pr * pt * (g + ps) * 0.125;
-    dt = c * pr * ps * (g + pt) * 0.125;
-    return;
-  }
-  // mid-edge nodes: 8..11 top (t = +1), 12..15 bottom (t = -1), 16..19 (t = 0)
-  const int e = k - 8;
-  const int grp = e >> 2, idx = e & 3;
-  // position on the top ring of each group: 9 (r=0,s=+), 10 (r=-,s=0), 11 (r=0,s=-), 12 (r=+,s=0)
-  if (grp < 2) {
-    const double c = grp == 0 ? 1.0 : -1.0, pt = 1.0 + c * t;
-    if ((idx & 1) == 0) {  // r = 0 edge, s = +-1
-      const double b = idx == 0 ? 1.0 : -1.0, ps = 1.0 + b * s, mrr = 1.0 - r * r;
+  const double pr = 1.0 + r, ps = 1.0 + s, pt = 1.0 + t;
+  const double mr = 1.0 - r, ms = 1.0 - s, mt = 1.0 - t;
+  const double mrr = 1.0 - r * r, mss = 1.0 - s * s, mtt = 1.0 - t * t;
+  switch (k) {
+    case 0:
+      h = pr * ps * pt * (r + s + t - 2.0) * 0.125;
+      dr = ps * pt * (2.0 * r + s + t - 1.0) * 0.125;
+      ds = pr * pt * (r + 2.0 * s + t - 1.0) * 0.125;
+      dt = pr * ps * (r + s + 2.0 * t - 1.0) * 0.125;
+      break;
+    case 1:
+      h = mr * ps * pt * (-r + s + t - 2.0) * 0.125;
+      dr = ps * pt * (2.0 * r - s - t + 1.0) * 0.125;
+      ds = mr * pt * (-r + 2.0 * s + t - 1.0) * 0.125;
+      dt = mr * ps * (-r + s + 2.0 * t - 1.0) * 0.125;
+      break;
+    case 2:
+      h = mr * ms * pt * (-r - s + t - 2.0) * 0.125;
+      dr = ms * pt * (2.0 * r + s - t + 1.0) * 0.125;
+      ds = mr * pt * (r + 2.0 * s - t + 1.0) * 0.125;
+      dt = mr * ms * (-r - s + 2.0 * t - 1.0) * 0.125;
+      break;
+    case 3:
+      h = pr * ms * pt * (r - s + t - 2.0) * 0.125;
+      dr = ms * pt * (2.0 * r - s + t - 1.0) * 0.125;
+      ds = pr * pt * (-r + 2.0 * s - t + 1.0) * 0.125;
+      dt = pr * ms * (r - s + 2.0 * t - 1.0) * 0.125;
+      break;
+    case 4:
+      h = pr * ps * mt * (r + s - t - 2.0) * 0.125;
+      dr = ps * mt * (2.0 * r + s - t - 1.0) * 0.125;
+      ds = pr * mt * (r + 2.0 * s - t - 1.0) * 0.125;
+      dt = pr * ps * (-r - s + 2.0 * t + 1.0) * 0.125;
+      break;
+    case 5:
+      h = mr * ps * mt * (-r + s - t - 2.0) * 0.125;
+      dr = ps * mt * (2.0 * r - s + t + 1.0) * 0.125;
+      ds = mr * mt * (-r + 2.0 * s - t - 1.0) * 0.125;
+      dt = mr * ps * (r - s + 2.0 * t + 1.0) * 0.125;
+      break;
+    case 6:
+      h = mr * ms * mt * (-r - s - t - 2.0) * 0.125;
+      dr = ms * mt * (2.0 * r + s + t + 1.0) * 0.125;
+      ds = mr * mt * (r + 2.0 * s + t + 1.0) * 0.125;
+      dt = mr * ms * (r + s + 2.0 * t + 1.0) * 0.125;
+      break;
+    case 7:
+      h = pr * ms * mt * (r - s - t - 2.0) * 0.125;
+      dr = ms * mt * (2.0 * r - s - t - 1.0) * 0.125;
+      ds = pr * mt * (-r + 2.0 * s + t + 1.0) * 0.125;
+      dt = pr * ms * (-r + s + 2.0 * t + 1.0) * 0.125;
+      break;
+    case 8:
       h = mrr * ps * pt * 0.25;
-      dr = -2.0 * r * ps * pt * 0.25;
-      ds = b * mrr * pt * 0.25;
-      dt = c * mrr * ps * 0.25;
-    } else {  // s = 0 edge, r = -+1
-      const double a = idx == 1 ? -1.0 : 1.0, pr = 1.0 + a * r, mss = 1.0 - s * s;
+      dr = -r * ps * pt * 0.50;
+      ds = mrr * pt * 0.25;
+      dt = mrr * ps * 0.25;
+      break;
+    case 9:
+      h = mr * mss * pt * 0.25;
+      dr = -mss * pt * 0.25;
+      ds = -s * mr * pt * 0.50;
+      dt = mr * mss * 0.25;
+      break;
+    case 10:
+      h = mrr * ms * pt * 0.25;
+      dr = -r * ms * pt * 0.50;
+      ds = -mrr * pt * 0.25;
+      dt = mrr * ms * 0.25;
+      break;
+    case 11:
       h = pr * mss * pt * 0.25;
-      dr = a * mss * pt * 0.25;
-      ds = -2.0 * s * pr * pt * 0.25;
-      dt = c * pr * mss * 0.25;
-    }
-  } else {  // vertical edges under vertices 1..4
-    const double a = c_rn[idx], b = c_sn[idx];
-    const double pr = 1.0 + a * r, ps = 1.0 + b * s, mtt = 1.0 - t * t;
-    h = pr * ps * mtt * 0.25;
-    dr = a * ps * mtt * 0.25;
-    ds = b * pr * mtt * 0.25;
-    dt = -2.0 * t * pr * ps * 0.25;
+      dr = mss * pt * 0.25;
+      ds = -s * pr * pt * 0.50;
+      dt = pr * mss * 0.25;
+      break;
+    case 12:
+      h = mrr * ps * mt * 0.25;
+      dr = -r * ps * mt * 0.50;
+      ds = mrr * mt * 0.25;
+      dt = -mrr * ps * 0.25;
+      break;
+    case 13:
+      h = mr * mss * mt * 0.25;
+      dr = -mss * mt * 0.25;
+      ds = -s * mr * mt * 0.50;
+      dt = -mr * mss * 0.25;
+      break;
+    case 14:
+      h = mrr * ms * mt * 0.25;
+      dr = -r * ms * mt * 0.50;
+      ds = -mrr * mt * 0.25;
+      dt = -mrr * ms * 0.25;
+      break;
+    case 15:
+      h = pr * mss * mt * 0.25;
+      dr = mss * mt * 0.25;
+      ds = -s * pr * mt * 0.50;
+      dt = -pr * mss * 0.25;
+      break;
+    case 16:
+      h = pr * ps * mtt * 0.25;
+      dr = ps * mtt * 0.25;
+      ds = pr * mtt * 0.25;
+      dt = -t * pr * ps * 0.50;
+      break;
+    case 17:
+      h = mr * ps * mtt * 0.25;
+      dr = -ps * mtt * 0.25;
+      ds = mr * mtt * 0.25;
+      dt = -t * mr * ps * 0.50;
+      break;
+    case 18:
+      h = mr * ms * mtt * 0.25;
+      dr = -ms * mtt * 0.25;
+      ds = -mr * mtt * 0.25;
+      dt = -t * mr * ms * 0.50;
+      break;
+    case 19:
+      h = pr * ms * mtt * 0.25;
+      dr = ms * mtt * 0.25;
+      ds = -pr * mtt * 0.25;
+      dt = -t * pr * ms * 0.50;
+      break;
+    default:
+      h = dr = ds = dt = 0.0;
   }
 }
 

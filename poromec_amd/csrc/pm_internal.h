@@ -74,6 +74,14 @@ void pm_host_graph(PmHostMesh &m);
 void pm_host_rcm(PmHostMesh &m, int reorder);
 void pm_host_numeq(PmHostMesh &m);
 int pm_host_pattern(const PmHostMesh &m, PmHostPattern &p);
+
+// element-order assembly maps (see pm_host_ordered_maps)
+struct PmOrderedMaps {
+  std::vector<int64_t> mptr, msrc;  // per stored value: e*68*68 + a*68 + b
+  std::vector<int64_t> rptr, rsrc;  // per equation: e*68 + a
+  std::vector<int64_t> pptr, psrc;  // per node: e*8 + vertex
+};
+int pm_host_ordered_maps(const PmHostMesh &m, const PmHostPattern &p, PmOrderedMaps &o);
 int pm_host_pattern_from_csrc(int neq, int64_t nad, const int64_t *ia, const int *ja,
                               PmHostPattern &p);
 void pm_host_colour(PmHostMesh &m);
@@ -133,6 +141,11 @@ struct pm_ctx {
   int *d_colour_elems = nullptr;
   std::vector<int> colour_ptr;
   int *d_ic = nullptr;      // (nnode) elements incident to each vertex
+  // element-order assembly (reference summation order), built on first use
+  int64_t *d_omptr = nullptr, *d_omsrc = nullptr, *d_orptr = nullptr, *d_orsrc = nullptr;
+  int64_t *d_opptr = nullptr, *d_opsrc = nullptr;
+  double *d_ke = nullptr, *d_pe = nullptr, *d_pv = nullptr;  // element arrays
+  bool have_ordered_maps = false;
 
   // nodal loads
   int *d_nload = nullptr;   // (nnode, 4)

@@ -40,12 +40,14 @@ __global__ __launch_bounds__(PM_WG) void k_spmv(
   int64_t k0, k1;
   pm_chunk(nrn, lb, gridDim.x, k0, k1);
   double d0 = 0.0, d1 = 0.0;
+  // the next block's metadata is loaded while this block streams
+  PmBlockMeta nx = pm_block_meta(k0 + grp, k1, colptr, valptr, nr);
   for (int64_t k = k0 + grp; k < k1; k += GROUPS) {
-    const int64_t cp = colptr[k];
-    const int c = (int)(colptr[k + 1] - cp);
-    const int n = nr[k];
+    const PmBlockMeta cur = nx;
+    nx = pm_block_meta(k + GROUPS, k1, colptr, valptr, nr);
+    const int n = cur.n;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    pm_spmv_block<VT, LANES, UNR>(lane, cp, c, valptr[k], n, ncol, val, x, a);
+    pm_spmv_block<VT, LANES, UNR>(lane, cur.cp, cur.c, cur.vp, n, ncol, val, x, a);
     if (lane == 0) {
       const int f = fr[k];
       for (int q = 0; q < n; q++) {

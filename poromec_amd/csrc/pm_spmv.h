@@ -16,6 +16,26 @@
 static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 32, 32, 16};
 static constexpr int pm_spmv_unr[PM_SPMV_NCFG] = {1, 2, 4, 2, 4, 4};
 
+struct PmBlockMeta {
+  int64_t cp, vp;
+  int c, n;
+};
+
+// metadata of node block k (an empty block past the end k1)
+__device__ __forceinline__ PmBlockMeta pm_block_meta(int64_t k, int64_t k1,
+                                                     const int64_t *__restrict__ colptr,
+                                                     const int64_t *__restrict__ valptr,
+                                                     const int *__restrict__ nr) {
+  PmBlockMeta m{0, 0, 0, 0};
+  if (k < k1) {
+    m.cp = colptr[k];
+    m.c = (int)(colptr[k + 1] - m.cp);
+    m.vp = valptr[k];
+    m.n = nr[k];
+  }
+  return m;
+}
+
 template <int LANES>
 __device__ __forceinline__ double pm_group_sum(double v) {
 #pragma unroll

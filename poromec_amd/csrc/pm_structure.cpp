@@ -480,3 +480,94 @@ void pm_host_colour(PmHostMesh &m) {
   std::vector<int> fill(m.colour_ptr.begin(), m.colour_ptr.end() - 1);
   for (int e = 0; e < m.numel; e++) m.colour_elems[fill[colour[e]]++] = e;
 }
+
+// ---------------------------------------------------------------------------
+// Element-order assembly maps (reference summation order).  The reference
+// adds element contributions in element order: al(k,l) += s(i,j) for the
+// local pair whose row equation is the larger (csr, Assbly.f:218-304),
+// ad(k) += s(i,i), b(k) -= p(i), and per vertex dporo += p (porosity_form).
+// For every stored position of the node-block matrix (both triangles take
+// the lower-oriented element entry), every equation and every node these
+// maps list the element-array entries that add into it, in element order.
+// ---------------------------------------------------------------------------
+static inline int64_t block_pos(const PmHostPattern &p, int r, int c) {
+  const int k = p.eq_block[r];
+  const int64_t cp = p.colptr[k];
+  const int cnt = (int)(p.colptr[k + 1] - cp);
+  const int *cols = p.ncol.data() + cp;
+  const int j = (int)(std::lower_bound(cols, cols + cnt, c) - cols);
+  if (j >= cnt || cols[j] != c) return -1;
+  return p.valptr[k] + (int64_t)(r - p.fr[k]) * cnt + j;
+}
+
+int pm_host_ordered_maps(const PmHostMesh &m, const PmHostPattern &p, PmOrderedMaps &o) {
+  const int numel = m.numel, nst = PM_NST;
+  const int64_t nnz = p.valptr[p.nrn];
+  std::vector<int> eqs((size_t)numel * nst);
+  for (int e = 0; e < numel; e++) {
+    const int *ix = m.ix20.data() + (size_t)e * 21;
+    for (int a = 0; a < nst; a++) {
+      const int node = a < 60 ? ix[a / 3] : ix[a - 60];
+      const int dof = a < 60 ? a % 3 : 3;
+      eqs[(size_t)e * nst + a] = node > 0 ? m.id[4 * (node - 1) + dof] - 1 : -1;
+    }
+  }
+  // matrix: count, then fill, element by element
+  std::vector<int64_t> cnt((size_t)nnz + 1, 0);
+  for (int pass = 0; pass < 2; pass++) {
+    std::vector<int64_t> fill;
+    if (pass == 1) {
+      o.mptr.assign((size_t)nnz + 1, 0);
+      for (int64_t t = 0; t < nnz; t++) o.mptr[t + 1] = o.mptr[t] + cnt[t];
+      o.msrc.assign((size_t)o.mptr[nnz], 0);
+      fill.assign(o.mptr.begin(), o.mptr.end() - 1);
+    }
+    for (int e = 0; e < numel; e++) {
+      const int *q = eqs.data() + (size_t)e * nst;
+      for (int a = 0; a < nst; a++) {
+        if (q[a] < 0) continue;
+        for (int b = 0; b < nst; b++) {
+          if (q[b] < 0 || q[b] > q[a]) continue;
+          const int64_t src = (int64_t)e * nst * nst + (int64_t)a * nst + b;
+          const int64_t p1 = block_pos(p, q[a], q[b]);
+          const int64_t p2 = q[a] == q[b] ? -1 : block_pos(p, q[b], q[a]);
+          if (p1 < 0 || (q[a] != q[b] && p2 < 0)) return PM_ERR_STATE;
+          if (pass == 0) {
+            cnt[p1]++;
+            if (p2 >= 0) cnt[p2]++;
+          } else {
+            o.msrc[fill[p1]++] = src;
+            if (p2 >= 0) o.msrc[fill[p2]++] = src;
+          }
+        }
+      }
+    }
+  }
+  // right-hand side per equation
+  std::vector<int64_t> rc((size_t)m.neq, 0);
+  for (size_t i = 0; i < eqs.size(); i++)
+    if (eqs[i] >= 0) rc[eqs[i]]++;
+  o.rptr.assign((size_t)m.neq + 1, 0);
+  for (int r = 0; r < m.neq; r++) o.rptr[r + 1] = o.rptr[r] + rc[r];
+  o.rsrc.assign((size_t)o.rptr[m.neq], 0);
+  std::vector<int64_t> rf(o.rptr.begin(), o.rptr.end() - 1);
+  for (size_t i = 0; i < eqs.size(); i++)
+    if (eqs[i] >= 0) o.rsrc[rf[eqs[i]]++] = (int64_t)i;
+  // porosity per node (the 8 vertices of every element)
+  std::vector<int64_t> pc((size_t)m.nnode, 0);
+  for (int e = 0; e < numel; e++)
+    for (int v = 0; v < 8; v++) {
+      const int node = m.ix20[(size_t)e * 21 + v];
+      if (node > 0) pc[node - 1]++;
+    }
+  o.pptr.assign((size_t)m.nnode + 1, 0);
+  for (int n = 0; n < m.nnode; n++) o.pptr[n + 1] = o.pptr[n] + pc[n];
+  o.psrc.assign((size_t)o.pptr[m.nnode], 0);
+  std::vector<int64_t> pf(o.pptr.begin(), o.pptr.end() - 1);
+  for (int e = 0; e < numel; e++)
+    for (int v = 0; v < 8; v++) {
+      const int node = m.ix20[(size_t)e * 21 + v];
+      if (node > 0) o.psrc[pf[node - 1]++] = (int64_t)e * 8 + v;
+    }
+  return PM_OK;
+}

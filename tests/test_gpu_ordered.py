@@ -94,3 +94,67 @@ def test_ordered_solvers_bitwise(request, system, solver, precond, tol):
     cs.close()
     assert info["iters"] == it_o, (info["iters"], it_o)
     assert np.array_equal(x, x_o), np.abs(x - x_o).max() / np.abs(x_o).max()
+
+
+def _gravity_box():
+    d = P.heterogeneous_box_deck(3, seed=3, sigma_lnk=0.8, corr_len=2.0, dt=0.05, nsteps=4)
+    d.gravity = np.array([0.0, 0.0, -1.0])
+    d.e[:, 5] = 2000.0
+    d.e[:, 6] = 1000.0
+    return d
+
+
+@pytest.mark.parametrize("which", ["column", "gravity_box"])
+def test_ordered_assembly_bitwise(which):
+    """isw 2 matrix and right-hand side, and isw 4 residual (face loads,
+    gravity), summed in element order: equal to the reference's bit for bit."""
+    from poromec_amd.solver import MefSolver
+    d = P.column_deck(2, 2, 6, dt=0.01, nsteps=6) if which == "column" else _gravity_box()
+    st = O.build_structure(d)
+    s = MefSolver(d)
+    s.set_ordered(True)
+    m = O.OracleModel(d, st)
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal((st["nnode"], 4)) * 1e-3
+    dp = rng.standard_normal(st["nnode"]) * 1e-4
+    u[d.nnodev:, 3] = 0.0
+    dp[d.nnodev:] = 0.0
+    m.u[:] = u
+    m.dp[:] = dp
+    s.set_state(u=u, u0=np.zeros_like(u), dp=dp, t=0.0)
+    for isw in (2, 4):
+        b0 = rng.standard_normal(st["neq"])
+        m.b[:] = b0
+        m.env.t = 0.5
+        m.assemble(isw=isw, lhs=(isw == 2), rhs=True)
+        s.set_vectors(b=b0)
+        s.pform(isw, isw == 2, True, d.dt, 0.5, target="b")
+        _, b, _ = s.vectors()
+        assert np.array_equal(b, m.b), (isw, np.abs(b - m.b).max())
+        if isw == 2:
+            ia, ja, ad, al = s.export_csrc()
+            assert np.array_equal(ad, m.ad), np.abs(ad - m.ad).max()
+            assert np.array_equal(al, m.al[: st["nad"]]), np.abs(al - m.al[: st["nad"]]).max()
+    s.close()
+
+
+@pytest.mark.parametrize("which", ["column", "gravity_box"])
+def test_ordered_march_bitwise(which):
+    """The whole backward-Euler march in the reference's order: the oracle's
+    nonlinear and Krylov iteration counts exactly, and u, dp and porosity
+    bit for bit after every step."""
+    from poromec_amd.solver import MefSolver, SolverConfig
+    d = P.column_deck(2, 2, 8, dt=0.01, nsteps=6) if which == "column" else _gravity_box()
+    st = O.build_structure(d)
+    m = O.OracleModel(d, st)
+    s = MefSolver(d)
+    s.set_ordered(True)
+    for k in range(4):
+        io = m.step()
+        ig = s.step(SolverConfig())
+        assert (io["nlit"], list(io["iters"])) == (ig["nlit"], ig["iters"]), (k, io, ig)
+        g = s.state()
+        assert np.array_equal(g["u"], m.u), (k, np.abs(g["u"] - m.u).max())
+        assert np.array_equal(g["dp"], m.dp), k
+        assert np.array_equal(g["porosity"], m.porosity), k
+    s.close()
