@@ -118,6 +118,93 @@ def cpu_baseline(n_sample, n_bench_numel, bench_nnz, gpu_iters_per_step, seed=0)
     }
 
 
+def batch_form(args):
+    """The batching form: nbox independent boxes (n^3 cells, n ~ U[nmin, nmax])
+    as one block-diagonal system, every box with its own stop tests; K steps
+    of FP64 and of FP32+IR.  Prints one JSON line (box-steps/s)."""
+    from poromec_amd import problem as P
+    from poromec_amd.solver import MefSolver, SolverConfig
+
+    decks = P.batch_decks(nbox=args.nbox, seed=args.seed, nmin=args.nmin, nmax=args.nmax)
+    big, off = P.concat_decks(decks)
+    t0 = time.perf_counter()
+    s = MefSolver(big)
+    s.batch_define(off)
+    setup_s = time.perf_counter() - t0
+    s.pform(2, True, False, big.dt, big.dt)
+    shapes = {name: s.tune_spmv(use32, reps=5)[0] for use32, name in ((False, "fp64"), (True, "fp32"))}
+    log(f"batch: {args.nbox} boxes, neq {s.neq}, setup {setup_s:.1f} s, shapes {shapes}")
+    out = {}
+    for name, cfg in (("fp64", SolverConfig(tol=1e-10)),
+                      ("fp32ir", SolverConfig(tol=1e-10, precision="fp32ir"))):
+        z4, zn = np.zeros((s.nnode, 4)), np.zeros(s.nnode)
+        s.set_state(u=z4, u0=z4, dp=zn, porosity=zn, t=0.0)
+        for _ in range(args.warmup):
+            s.step(cfg)
+        t0 = time.perf_counter()
+        its = []
+        for _ in range(args.steps):
+            s.step(cfg)
+            its.append(s.batch_iters().tolist())
+        wall = time.perf_counter() - t0
+        it = np.array(its)
+        out[name] = {"box_steps_per_s": args.nbox * args.steps / wall, "ms_per_step": 1e3 * wall / max(args.steps, 1),
+                     "iters_per_box_step": {"min": int(it.min()), "mean": float(it.mean()), "max": int(it.max())}}
+        log(f"  {name}: {out[name]}")
+    print(json.dumps({"form": "batch", "nbox": args.nbox, "n_range": [args.nmin, args.nmax],
+                      "neq": s.neq, "steps": args.steps, "warmup": args.warmup,
+                      "setup_s": setup_s, "spmv_shapes": shapes, **out}), flush=True)
+
+
+def adjoint_form(args):
+    """The differentiation form: misfit of pore-pressure histories at nsens
+    seeded sensor nodes against targets from a second seeded field, over
+    nsteps steps; the adjoint gradient w.r.t. per-element log k and log E
+    (one material per element) timed, then checked against central finite
+    differences along ndir seeded random directions."""
+    from poromec_amd import problem as P
+    from poromec_amd.adjoint import MarchMisfit
+    from poromec_amd.solver import MefSolver, SolverConfig
+
+    n = args.n
+    deck = P.heterogeneous_box_deck(n, seed=args.seed, dt=0.1, nsteps=args.nsteps,
+                                    corr_len=min(8.0, n / 2))
+    cfg = SolverConfig(tol=1e-13, maxit=200000)
+    rng = np.random.default_rng(args.seed + 1)
+    probe = MefSolver(deck)
+    _, idm, _ = probe.numbering()
+    free_p = np.nonzero(idm[: deck.nnodev, 3] > 0)[0]
+    sens = [(int(v), 3) for v in rng.choice(free_p, size=min(args.nsens, len(free_p)), replace=False)]
+    tdeck = P.heterogeneous_box_deck(n, seed=args.seed + 100, dt=0.1, nsteps=args.nsteps,
+                                     corr_len=min(8.0, n / 2))
+    probe.set_materials(tdeck)
+    obs = np.zeros((args.nsteps, len(sens)))
+    for k in range(args.nsteps):
+        probe.step(cfg)
+        u = probe.state()["u"]
+        obs[k] = [u[a, b] for a, b in sens]
+    probe.close()
+    m = MarchMisfit(deck, sens, obs, nsteps=args.nsteps, cfg=cfg, ckpt_every=args.ckpt)
+    lE, lk = (t.numpy() for t in m.params())
+    t0 = time.perf_counter()
+    J, gE, gk = m._evaluate(lE, lk)
+    t_adj = time.perf_counter() - t0
+    g = np.concatenate([gE, gk])
+    h, errs = args.fd_h, []
+    for _ in range(args.ndir):
+        v = rng.standard_normal(2 * len(lE))
+        jp = m._evaluate(lE + h * v[: len(lE)], lk + h * v[len(lE):])[0]
+        jm = m._evaluate(lE - h * v[: len(lE)], lk - h * v[len(lE):])[0]
+        fd = (jp - jm) / (2 * h)
+        errs.append(abs(fd - g @ v) / max(abs(fd), 1e-300))
+        log(f"  direction {len(errs)}: fd {fd:.12e} adjoint {g @ v:.12e} rel {errs[-1]:.2e}")
+    print(json.dumps({"form": "adjoint", "n": n, "numel": deck.numel, "neq": m.solver.neq,
+                      "nsteps": args.nsteps, "nsens": len(sens), "ckpt_every": args.ckpt,
+                      "J": J, "forward_plus_adjoint_s": t_adj, "fd_h": h, "ndir": args.ndir,
+                      "fd_rel_err_max": max(errs) if errs else None,
+                      "fd_rel_err_median": float(np.median(errs)) if errs else None}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +215,16 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--cpu-sample-n", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", choices=["precision", "batch", "adjoint"], default="precision",
+                    help="precision (default, the metric's workload), batch or adjoint form")
+    ap.add_argument("--nbox", type=int, default=256)
+    ap.add_argument("--nmin", type=int, default=12)
+    ap.add_argument("--nmax", type=int, default=32)
+    ap.add_argument("--nsteps", type=int, default=50, help="adjoint form: march length")
+    ap.add_argument("--nsens", type=int, default=64)
+    ap.add_argument("--ckpt", type=int, default=10)
+    ap.add_argument("--ndir", type=int, default=20)
+    ap.add_argument("--fd-h", type=float, default=1e-6)
     ap.add_argument("--spmv-only", action="store_true",
                     help="set up, then only time the FP64 / FP32 SpMV and a copy kernel "
                          "(the run profiled with hardware counters)")
@@ -136,6 +233,11 @@ def main():
         print("bench.py runs on one GPU only: this path has no multi-GPU form "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.form == "batch":
+        return batch_form(args)
+    if args.form == "adjoint":
+        return adjoint_form(args)
 
     from poromec_amd import problem as P
     from poromec_amd.solver import MefSolver, SolverConfig

@@ -186,10 +186,10 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
   switch (cfg) {
     PM_BSPMV_CASE(0, 64, 1)
     PM_BSPMV_CASE(1, 64, 2)
-    PM_BSPMV_CASE(2, 64, 4)
-    PM_BSPMV_CASE(3, 32, 2)
+    PM_BSPMV_CASE(2, 64, 3)
+    PM_BSPMV_CASE(3, 64, 5)
     PM_BSPMV_CASE(4, 32, 4)
-    default: PM_BSPMV_CASE(5, 16, 4)
+    default: PM_BSPMV_CASE(5, 32, 8)
   }
 #undef PM_BSPMV_CASE
 }

@@ -108,10 +108,10 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
   switch (cfg) {
     PM_SPMV_CASE(0, 64, 1)
     PM_SPMV_CASE(1, 64, 2)
-    PM_SPMV_CASE(2, 64, 4)
-    PM_SPMV_CASE(3, 32, 2)
+    PM_SPMV_CASE(2, 64, 3)
+    PM_SPMV_CASE(3, 64, 5)
     PM_SPMV_CASE(4, 32, 4)
-    default: PM_SPMV_CASE(5, 16, 4)
+    default: PM_SPMV_CASE(5, 32, 8)
   }
 #undef PM_SPMV_CASE
   return g;
