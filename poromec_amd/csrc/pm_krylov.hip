@@ -30,25 +30,32 @@ template <typename VT, int LANES, int UNR, int NV, bool SCALE_M>
 __global__ __launch_bounds__(PM_WG) void k_spmv(
     int nrn, const int *__restrict__ fr, const int *__restrict__ nr,
     const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
-    const int *__restrict__ ncol, const VT *__restrict__ val, const double *__restrict__ x,
+    const int *__restrict__ ncol, const unsigned int *__restrict__ runs,
+    const int64_t *__restrict__ runptr, const VT *__restrict__ val, const double *__restrict__ x,
     double *__restrict__ y, const double *__restrict__ w, const double *__restrict__ m,
     double *partials, const PmScalars *S) {
   if (S != nullptr && S->done) return;
   constexpr int GROUPS = PM_WG / LANES;
+  __shared__ int s_cols[GROUPS][PM_SPMV_MAXC];
   const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
   const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
   int64_t k0, k1;
   pm_chunk(nrn, lb, gridDim.x, k0, k1);
   double d0 = 0.0, d1 = 0.0;
   // the next block's metadata is loaded while this block streams
-  PmBlockMeta nx = pm_block_meta(k0 + grp, k1, colptr, valptr, nr);
+  PmBlockMeta nx = pm_block_meta(k0 + grp, k1, colptr, valptr, nr, runptr);
   for (int64_t k = k0 + grp; k < k1; k += GROUPS) {
     const PmBlockMeta cur = nx;
-    nx = pm_block_meta(k + GROUPS, k1, colptr, valptr, nr);
+    nx = pm_block_meta(k + GROUPS, k1, colptr, valptr, nr, runptr);
     const int n = cur.n;
+    const int *cols = ncol + cur.cp;
+    if (runptr != nullptr && cur.c <= PM_SPMV_MAXC) {
+      pm_expand_runs<LANES>(lane, runs + cur.rp, cur.nrun, s_cols[grp]);
+      cols = s_cols[grp];
+    }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    pm_spmv_block<VT, LANES, UNR>(lane, cur.cp, cur.c, cur.vp, n, ncol, val, x, a);
-    if (lane == 0) {
+    pm_spmv_block<VT, LANES, UNR>(lane, cols, cur.c, cur.vp, n, val, x, a);
+    if (lane == LANES - 1) {
       const int f = fr[k];
       for (int q = 0; q < n; q++) {
         double yq = a[q];
@@ -102,8 +109,8 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
 #define PM_SPMV_CASE(id, L, U)                                                                  \
   case id:                                                                                      \
     hipLaunchKernelGGL((k_spmv<VT, L, U, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream,   \
-                       A.nrn, A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w, m,          \
-                       c->d_partials, S);                                                       \
+                       A.nrn, A.fr, A.nr, A.colptr, A.valptr, A.ncol, A.runs, A.runptr, val,    \
+                       x, y, w, m, c->d_partials, S);                                           \
     break;
   switch (cfg) {
     PM_SPMV_CASE(0, 64, 1)
@@ -1034,7 +1041,7 @@ int pm_solve_dev(pm_ctx *c, const pm_solver_cfg *cfg, const double *b, double *x
       std::vector<double> r2, b2;
       std::vector<int> active(nbox, 1);
       const int maxouter = cfg->ir_maxouter > 0 ? cfg->ir_maxouter : 10;
-      const double inner = cfg->ir_inner_tol > 0 ? cfg->ir_inner_tol : 1e-6;
+      const double inner = cfg->ir_inner_tol > 0 ? cfg->ir_inner_tol : 1e-5;
       bool all = false;
       for (int o = 0; o < maxouter && !all; o++) {
         pm_solve_info sub;
@@ -1096,7 +1103,7 @@ int pm_solve_dev(pm_ctx *c, const pm_solver_cfg *cfg, const double *b, double *x
     PM_TRY(read_red(c, nullptr, &bnorm2));
     double rel = 1.0;
     int maxouter = cfg->ir_maxouter > 0 ? cfg->ir_maxouter : 10;
-    double inner = cfg->ir_inner_tol > 0 ? cfg->ir_inner_tol : 1e-6;
+    double inner = cfg->ir_inner_tol > 0 ? cfg->ir_inner_tol : 1e-5;
     for (int o = 0; o < maxouter; o++) {
       pm_solve_info sub;
       memset(&sub, 0, sizeof(sub));

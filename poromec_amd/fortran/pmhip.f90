@@ -21,7 +21,7 @@ module pmhip
     integer(c_int) :: nkrylov = 150
     real(c_double) :: tol = 1.0d-11
     integer(c_int) :: precision = 0
-    real(c_double) :: ir_inner_tol = 1.0d-6
+    real(c_double) :: ir_inner_tol = 1.0d-5
     integer(c_int) :: ir_maxouter = 10
     integer(c_int) :: maxnlit = 500
     real(c_double) :: nltol = 1.0d-4

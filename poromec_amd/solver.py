@@ -36,7 +36,7 @@ class SolverConfig:
     maxit: int = 50000
     ngram: int = 150
     precision: str = "fp64"     # "fp64" or "fp32ir"
-    ir_inner_tol: float = 1.0e-6
+    ir_inner_tol: float = 1.0e-5
     ir_maxouter: int = 10
     maxnlit: int = 500
     nltol: float = 1.0e-4
