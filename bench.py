@@ -255,8 +255,11 @@ def main():
 
     if args.spmv_only:
         solver.pform(2, True, False, deck.dt, deck.dt)
-        solver.tune_spmv(False, reps=5)
-        solver.tune_spmv(True, reps=5)
+        shapes = {}
+        for use32, name in ((False, "fp64"), (True, "fp32")):
+            best, ms = solver.tune_spmv(use32, reps=5)
+            shapes[name] = {"best": best, "ms": [round(v, 4) for v in ms]}
+            log(f"SpMV shapes {name}: " + ", ".join(f"{v:.3f}" for v in ms) + f" ms -> shape {best}")
         ms64 = solver.bench_spmv(False, args.spmv_reps)
         ms32 = solver.bench_spmv(True, args.spmv_reps)
         copy_bytes = 1 << 30
@@ -265,7 +268,8 @@ def main():
                           "spmv_fp64_bytes": spmv_bytes(info, 8),
                           "spmv_fp32_bytes": spmv_bytes(info, 4), "copy_ms": msc,
                           "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes,
-                          "copy_GBps": 2 * copy_bytes / msc / 1e6, **info}), flush=True)
+                          "copy_GBps": 2 * copy_bytes / msc / 1e6, "tuned_shapes": shapes,
+                          **info}), flush=True)
         return
 
     # SpMV shape tuner on this matrix (lanes per node block x columns in flight)

@@ -72,8 +72,6 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, const int *__rest
                                                   const int64_t *__restrict__ colptr,
                                                   const int64_t *__restrict__ valptr,
                                                   const int *__restrict__ ncol,
-                                                  const unsigned int *__restrict__ runs,
-                                                  const int64_t *__restrict__ runptr,
                                                   const VT *__restrict__ val,
                                                   const double *__restrict__ x,
                                                   double *__restrict__ y,
@@ -83,26 +81,20 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, const int *__rest
   const PmScalars *Sb = S + box;
   if (Sb->done) return;
   constexpr int GROUPS = PM_WG / LANES;
-  __shared__ int s_cols[GROUPS][PM_SPMV_MAXC];
   const int gb = B.wg0[box], gc = B.wg0[box + 1] - gb;
   const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
   const int64_t nb0 = B.blk0[box], nblk = B.blk0[box + 1] - nb0;
   int64_t k0, k1;
   pm_chunk(nblk, blockIdx.x - gb, gc, k0, k1);
   double d0 = 0.0, d1 = 0.0;
-  PmBlockMeta nx = pm_block_meta(nb0 + k0 + grp, nb0 + k1, colptr, valptr, nr, runptr);
+  PmBlockMeta nx = pm_block_meta(nb0 + k0 + grp, nb0 + k1, colptr, valptr, nr);
   for (int64_t kk = k0 + grp; kk < k1; kk += GROUPS) {
     const int64_t k = nb0 + kk;
-    const PmBlockMeta cur = nx;
-    nx = pm_block_meta(k + GROUPS, nb0 + k1, colptr, valptr, nr, runptr);
+    const PmBlockMeta cur = pm_uniform<LANES>(nx);
+    nx = pm_block_meta(k + GROUPS, nb0 + k1, colptr, valptr, nr);
     const int n = cur.n;
-    const int *cols = ncol + cur.cp;
-    if (runptr != nullptr && cur.c <= PM_SPMV_MAXC) {
-      pm_expand_runs<LANES>(lane, runs + cur.rp, cur.nrun, s_cols[grp]);
-      cols = s_cols[grp];
-    }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    pm_spmv_block<VT, LANES, UNR>(lane, cols, cur.c, cur.vp, n, val, x, a);
+    pm_spmv_block<VT, LANES, UNR>(lane, cur, ncol, val, x, a);
     if (lane == LANES - 1) {
       const int f = fr[k];
       for (int q = 0; q < n; q++) {
@@ -188,16 +180,16 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
 #define PM_BSPMV_CASE(id, L, U)                                                                 \
   case id:                                                                                      \
     hipLaunchKernelGGL((k_spmv_b<VT, L, U, NV>), dim3(B->G), dim3(PM_WG), 0, c->stream,        \
-                       batch_args(B), A.fr, A.nr, A.colptr, A.valptr, A.ncol, A.runs,          \
-                       A.runptr, val, x, y, w, B->d_S);                                         \
+                       batch_args(B), A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w,    \
+                       B->d_S);                                                                 \
     break;
   switch (cfg) {
     PM_BSPMV_CASE(0, 64, 1)
     PM_BSPMV_CASE(1, 64, 2)
     PM_BSPMV_CASE(2, 64, 3)
-    PM_BSPMV_CASE(3, 64, 5)
-    PM_BSPMV_CASE(4, 32, 4)
-    default: PM_BSPMV_CASE(5, 32, 8)
+    PM_BSPMV_CASE(3, 64, 4)
+    PM_BSPMV_CASE(4, 64, 5)
+    default: PM_BSPMV_CASE(5, 32, 4)
   }
 #undef PM_BSPMV_CASE
 }

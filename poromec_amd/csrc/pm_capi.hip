@@ -49,8 +49,6 @@ static void free_matrix(PmDevMatrix &A) {
   FREE(A.val32);
   FREE(A.diag);
   FREE(A.diagpos);
-  FREE(A.runs);
-  FREE(A.runptr);
   A = PmDevMatrix();
 }
 
@@ -175,27 +173,6 @@ int pm_upload_pattern(pm_ctx *c, const PmHostPattern &p) {
   PM_TRY(dupload(c, &A.valptr, p.valptr.data(), p.valptr.size()));
   PM_TRY(dupload(c, &A.ncol, p.ncol.data(), p.ncol.size()));
   PM_TRY(dupload(c, &A.eq_block, p.eq_block.data(), p.eq_block.size()));
-  // runs of consecutive columns (equations < 2^26 fit the packed form)
-  if (p.neq < (1 << 26)) {
-    std::vector<unsigned int> runs;
-    std::vector<int64_t> runptr((size_t)p.nrn + 1, 0);
-    runs.reserve((size_t)(A.ncols / 4 + p.nrn));
-    for (int k = 0; k < p.nrn; k++) {
-      int64_t j = p.colptr[k];
-      const int64_t e = p.colptr[k + 1];
-      while (j < e) {
-        const int first = p.ncol[j];
-        int len = 1;
-        while (j + len < e && len < 64 && p.ncol[j + len] == first + len) len++;
-        runs.push_back(((unsigned int)first << 6) | (unsigned int)(len - 1));
-        j += len;
-      }
-      runptr[k + 1] = (int64_t)runs.size();
-    }
-    A.nruns = (int64_t)runs.size();
-    PM_TRY(dupload(c, &A.runs, runs.data(), runs.size()));
-    PM_TRY(dupload(c, &A.runptr, runptr.data(), runptr.size()));
-  }
   PM_TRY(dalloc(&A.val, (size_t)A.nnz));
   PM_CHECK_HIP(hipMemsetAsync(A.val, 0, sizeof(double) * (size_t)A.nnz, c->stream));
   PM_TRY(dalloc(&A.diag, (size_t)A.neq));

@@ -103,11 +103,6 @@ struct PmDevMatrix {
   float *val32 = nullptr;
   double *diag = nullptr;  // ad
   int64_t *diagpos = nullptr; // position of the diagonal entry of each row
-  // the same column lists as runs of consecutive equations, for the SpMV:
-  // run = (first << 6) | (length - 1), at most 64 long; runptr per block
-  unsigned int *runs = nullptr;
-  int64_t *runptr = nullptr;
-  int64_t nruns = 0;
   bool have32 = false;
   bool values_valid = false;
 };

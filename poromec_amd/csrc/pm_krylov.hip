@@ -30,38 +30,40 @@ template <typename VT, int LANES, int UNR, int NV, bool SCALE_M>
 __global__ __launch_bounds__(PM_WG) void k_spmv(
     int nrn, const int *__restrict__ fr, const int *__restrict__ nr,
     const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
-    const int *__restrict__ ncol, const unsigned int *__restrict__ runs,
-    const int64_t *__restrict__ runptr, const VT *__restrict__ val, const double *__restrict__ x,
+    const int *__restrict__ ncol, const VT *__restrict__ val, const double *__restrict__ x,
     double *__restrict__ y, const double *__restrict__ w, const double *__restrict__ m,
     double *partials, const PmScalars *S) {
   if (S != nullptr && S->done) return;
   constexpr int GROUPS = PM_WG / LANES;
-  __shared__ int s_cols[GROUPS][PM_SPMV_MAXC];
   const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
   const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
   int64_t k0, k1;
   pm_chunk(nrn, lb, gridDim.x, k0, k1);
   double d0 = 0.0, d1 = 0.0;
   // the next block's metadata is loaded while this block streams
-  PmBlockMeta nx = pm_block_meta(k0 + grp, k1, colptr, valptr, nr, runptr);
+  PmBlockMeta nx = pm_block_meta(k0 + grp, k1, colptr, valptr, nr);
   for (int64_t k = k0 + grp; k < k1; k += GROUPS) {
-    const PmBlockMeta cur = nx;
-    nx = pm_block_meta(k + GROUPS, k1, colptr, valptr, nr, runptr);
+    const PmBlockMeta cur = pm_uniform<LANES>(nx);
+    nx = pm_block_meta(k + GROUPS, k1, colptr, valptr, nr);
     const int n = cur.n;
-    const int *cols = ncol + cur.cp;
-    if (runptr != nullptr && cur.c <= PM_SPMV_MAXC) {
-      pm_expand_runs<LANES>(lane, runs + cur.rp, cur.nrun, s_cols[grp]);
-      cols = s_cols[grp];
+    // the row data the block's epilogue needs, loaded before the column
+    // stream so that its latency hides behind it
+    const int f = fr[k];
+    double wq[4] = {0.0, 0.0, 0.0, 0.0}, mq[4] = {1.0, 1.0, 1.0, 1.0};
+    if (lane == LANES - 1) {
+      for (int q = 0; q < n; q++) {
+        if (NV >= 1) wq[q] = w[f + q];
+        if (SCALE_M) mq[q] = m[f + q];
+      }
     }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    pm_spmv_block<VT, LANES, UNR>(lane, cols, cur.c, cur.vp, n, val, x, a);
+    pm_spmv_block<VT, LANES, UNR>(lane, cur, ncol, val, x, a);
     if (lane == LANES - 1) {
-      const int f = fr[k];
       for (int q = 0; q < n; q++) {
         double yq = a[q];
-        if (SCALE_M) yq *= m[f + q];
+        if (SCALE_M) yq *= mq[q];
         y[f + q] = yq;
-        if (NV >= 1) d0 += yq * w[f + q];
+        if (NV >= 1) d0 += yq * wq[q];
         if (NV >= 2) d1 += yq * yq;
       }
     }
@@ -109,16 +111,16 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
 #define PM_SPMV_CASE(id, L, U)                                                                  \
   case id:                                                                                      \
     hipLaunchKernelGGL((k_spmv<VT, L, U, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream,   \
-                       A.nrn, A.fr, A.nr, A.colptr, A.valptr, A.ncol, A.runs, A.runptr, val,    \
-                       x, y, w, m, c->d_partials, S);                                           \
+                       A.nrn, A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w, m,          \
+                       c->d_partials, S);                                                       \
     break;
   switch (cfg) {
     PM_SPMV_CASE(0, 64, 1)
     PM_SPMV_CASE(1, 64, 2)
     PM_SPMV_CASE(2, 64, 3)
-    PM_SPMV_CASE(3, 64, 5)
-    PM_SPMV_CASE(4, 32, 4)
-    default: PM_SPMV_CASE(5, 32, 8)
+    PM_SPMV_CASE(3, 64, 4)
+    PM_SPMV_CASE(4, 64, 5)
+    default: PM_SPMV_CASE(5, 32, 4)
   }
 #undef PM_SPMV_CASE
   return g;

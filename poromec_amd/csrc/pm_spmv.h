@@ -8,74 +8,49 @@
 // them back to back before it uses any, which raises the number of bytes in
 // flight per wave (the SpMV is bound by them, not by instruction issue).
 // Each lane carries the partial sums of every row of its block; DPP moves
-// over the LANES lanes finish them.  Column indices come as runs of
-// consecutive equations, expanded into LDS per block.
+// over the LANES lanes finish them.
 #pragma once
 #include "pm_device.h"
 
 // (LANES, UNR) of each SpMV shape the tuner may pick
 #define PM_SPMV_NCFG 6
-static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 64, 32, 32};
-static constexpr int pm_spmv_unr[PM_SPMV_NCFG] = {1, 2, 3, 5, 4, 8};
-
-// columns of one node block held in LDS when it comes as runs
-#define PM_SPMV_MAXC 512
+static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 64, 64, 32};
+static constexpr int pm_spmv_unr[PM_SPMV_NCFG] = {1, 2, 3, 4, 5, 4};
 
 struct PmBlockMeta {
-  int64_t cp, vp, rp;
-  int c, n, nrun;
+  int64_t cp, vp;
+  int c, n;
 };
 
 // metadata of node block k (an empty block past the end k1)
 __device__ __forceinline__ PmBlockMeta pm_block_meta(int64_t k, int64_t k1,
                                                      const int64_t *__restrict__ colptr,
                                                      const int64_t *__restrict__ valptr,
-                                                     const int *__restrict__ nr,
-                                                     const int64_t *__restrict__ runptr) {
-  PmBlockMeta m{0, 0, 0, 0, 0, 0};
+                                                     const int *__restrict__ nr) {
+  PmBlockMeta m{0, 0, 0, 0};
   if (k < k1) {
     m.cp = colptr[k];
     m.c = (int)(colptr[k + 1] - m.cp);
     m.vp = valptr[k];
     m.n = nr[k];
-    if (runptr) {
-      m.rp = runptr[k];
-      m.nrun = (int)(runptr[k + 1] - m.rp);
-    }
   }
   return m;
 }
 
-__device__ __forceinline__ void pm_wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Expands the block's runs of consecutive columns (first << 6 | length - 1)
-// into scol[0 .. c): each lane takes one run, an inclusive scan over the
-// group gives its start.  Reading 4 bytes per run instead of per column cuts
-// the index stream about ninefold on these meshes (runs of ~9 columns).
+// A whole wavefront works on one block when LANES == 64: its metadata is
+// then wave-uniform, and moving it to scalar registers lets the loads use a
+// scalar base with 32-bit lane offsets (fewer vector registers, more waves).
 template <int LANES>
-__device__ __forceinline__ void pm_expand_runs(int lane, const unsigned int *__restrict__ rr,
-                                               int nrun, int *scol) {
-  int carry = 0;
-  for (int r0 = 0; r0 < nrun; r0 += LANES) {
-    const int r = r0 + lane;
-    const unsigned int w = r < nrun ? __builtin_nontemporal_load(rr + r) : 0u;
-    const int len = r < nrun ? (int)(w & 63u) + 1 : 0;
-    const int first = (int)(w >> 6);
-    int incl = len;
-#pragma unroll
-    for (int off = 1; off < LANES; off <<= 1) {
-      const int t = __shfl_up(incl, off, LANES);
-      if (lane >= off) incl += t;
-    }
-    const int start = carry + incl - len;
-    for (int t = 0; t < len; t++) scol[start + t] = first + t;
-    carry += __shfl(incl, LANES - 1, LANES);
+__device__ __forceinline__ PmBlockMeta pm_uniform(PmBlockMeta m) {
+  if (LANES == 64) {
+    m.cp = ((int64_t)__builtin_amdgcn_readfirstlane((int)(m.cp >> 32)) << 32) |
+           (unsigned int)__builtin_amdgcn_readfirstlane((int)m.cp);
+    m.vp = ((int64_t)__builtin_amdgcn_readfirstlane((int)(m.vp >> 32)) << 32) |
+           (unsigned int)__builtin_amdgcn_readfirstlane((int)m.vp);
+    m.c = __builtin_amdgcn_readfirstlane(m.c);
+    m.n = __builtin_amdgcn_readfirstlane(m.n);
   }
-  pm_wave_sync();
+  return m;
 }
 
 // one DPP move of a double (two 32-bit halves); lanes outside row_mask get 0
@@ -100,46 +75,50 @@ __device__ __forceinline__ double pm_group_sum(double v) {
   return v;
 }
 
-// partial row sums of node block (cp, c, vp, n), complete in the group's
-// last lane.  Each pass covers UNR * LANES columns: every lane issues its UNR
-// index loads and all value loads of the pass before the x gathers and the
-// products, so a wave keeps a whole block's streams in flight instead of one
-// 64-column slice at a time.  Past the end of the block (or of its rows) a
-// lane loads the block's last valid element again and multiplies by zero:
-// straight-line code, no exec-mask branches between the loads.
+// partial row sums of node block m, complete in the group's last lane.  Each
+// pass covers UNR * LANES columns: every lane issues its UNR index loads and
+// the value loads of the block's rows before the x gathers and the products,
+// so a wave keeps a whole block's streams in flight instead of one 64-column
+// slice at a time.  Past the end of the block a lane reloads the block's last
+// column, past its row count row 0, and selects zero after the load:
+// straight-line code, no branches between the loads.
 template <typename VT, int LANES, int UNR>
-__device__ __forceinline__ void pm_spmv_block(int lane, const int *cols, int c, int64_t vp, int n,
+__device__ __forceinline__ void pm_spmv_block(int lane, const PmBlockMeta &m,
+                                              const int *__restrict__ ncol,
                                               const VT *__restrict__ val,
                                               const double *__restrict__ x, double a[4]) {
-  const VT *v0 = val + vp;
-  const int64_t rs1 = n > 1 ? c : 0, rs2 = n > 2 ? 2 * (int64_t)c : 0, rs3 = n > 3 ? 3 * (int64_t)c : 0;
-  const double m1 = n > 1 ? 1.0 : 0.0, m2 = n > 2 ? 1.0 : 0.0, m3 = n > 3 ? 1.0 : 0.0;
+  const int *cols = ncol + m.cp;
+  const VT *v0 = val + m.vp;
+  const int c = m.c, n = m.n;
+  // rows past the block's count reload row 0 and are zeroed after the load
+  const int r1 = n > 1 ? c : 0, r2 = n > 2 ? 2 * c : 0, r3 = n > 3 ? 3 * c : 0;
   for (int base = 0; base < c; base += UNR * LANES) {  // same trip count in the group
-    const int j0 = base + lane;
     int col[UNR];
-    VT w[4][UNR];
-    double keep[UNR];
+    bool ok[UNR];
+    VT w0[UNR], w1[UNR], w2[UNR], w3[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; u++) {
-      const int j = j0 + u * LANES;
-      const int jj = j < c ? j : c - 1;
-      keep[u] = j < c ? 1.0 : 0.0;
-      col[u] = cols[jj];
-      const VT *v = v0 + jj;
-      w[0][u] = __builtin_nontemporal_load(v);
-      w[1][u] = __builtin_nontemporal_load(v + rs1);
-      w[2][u] = __builtin_nontemporal_load(v + rs2);
-      w[3][u] = __builtin_nontemporal_load(v + rs3);
+      const int j = base + lane + u * LANES;
+      ok[u] = j < c;
+      const int jj = ok[u] ? j : c - 1;
+      col[u] = __builtin_nontemporal_load(cols + jj);
+      w0[u] = __builtin_nontemporal_load(v0 + jj);
+      w1[u] = __builtin_nontemporal_load(v0 + r1 + jj);
+      w2[u] = __builtin_nontemporal_load(v0 + r2 + jj);
+      w3[u] = __builtin_nontemporal_load(v0 + r3 + jj);
     }
     double xv[UNR];
 #pragma unroll
-    for (int u = 0; u < UNR; u++) xv[u] = x[col[u]] * keep[u];
+    for (int u = 0; u < UNR; u++) {
+      const double xc = x[col[u]];
+      xv[u] = ok[u] ? xc : 0.0;
+    }
 #pragma unroll
     for (int u = 0; u < UNR; u++) {
-      a[0] += (double)w[0][u] * xv[u];
-      a[1] += (double)w[1][u] * (xv[u] * m1);
-      a[2] += (double)w[2][u] * (xv[u] * m2);
-      a[3] += (double)w[3][u] * (xv[u] * m3);
+      a[0] += (double)w0[u] * xv[u];
+      a[1] += (double)(n > 1 ? w1[u] : VT(0)) * xv[u];
+      a[2] += (double)(n > 2 ? w2[u] : VT(0)) * xv[u];
+      a[3] += (double)(n > 3 ? w3[u] : VT(0)) * xv[u];
     }
   }
 #pragma unroll
