@@ -31,6 +31,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "time-steps/s at 8.6M-dof Biot u-p (FP32+IR vs FP64 @1e-10); SpMV GB/s"
+# (lanes per node block, columns in flight per lane) of the SpMV shapes 0..5
+shape_lanes = (64, 64, 64, 64, 64, 32)
+shape_unr = (1, 2, 3, 4, 5, 4)
 HBM_PEAK_GBPS = 8000.0
 
 
@@ -38,12 +41,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def spmv_bytes(info, value_bytes):
+def spmv_bytes(info, value_bytes, fused_dot=False):
     """Bytes one SpMV launch must move in the node-block layout: every value
     once, every shared column index once, the per-block metadata (colptr,
-    valptr: 8 B each; first row, row count: 4 B each), x once, y once."""
+    valptr: 8 B each; first row, row count: 4 B each), x once, y once, and w
+    once when the solver's dot y.w is fused into it."""
     return (value_bytes * info["nnz"] + 4 * info["ncols"] + 24 * info["nblocks"]
-            + 16 * info["neq"])
+            + (24 if fused_dot else 16) * info["neq"])
 
 
 def csr_bytes(info):
@@ -52,7 +56,24 @@ def csr_bytes(info):
     return 12 * info["nnz"] + 8 * (info["neq"] + 1) + 16 * info["neq"]
 
 
-def run_form(solver, cfg, warmup, steps, keep_states=False):
+def measured_traffic(prec, shape, info):
+    """HBM bytes per launch of the SpMV at this shape and workload, from the
+    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by a
+    copy kernel of known size), or None when it was not measured for them."""
+    path = ROOT / "profiles" / "spmv_87cube_hbm_counters.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if not line.startswith("<!-- traffic "):
+            continue
+        kv = dict(t.split("=", 1) for t in line[len("<!-- traffic "):].rstrip(" ->").split())
+        if (kv.get("prec") == prec and int(kv.get("shape", -1)) == shape
+                and int(kv.get("neq", -1)) == info["neq"] and int(kv.get("nnz", -1)) == info["nnz"]):
+            return float(kv["bytes"])
+    return None
+
+
+def run_form(solver, cfg, warmup, steps, keep_states=False, live_every=16):
     from poromec_amd.solver import MefSolver  # noqa: F401
 
     z4 = np.zeros((solver.nnode, 4))
@@ -65,7 +86,7 @@ def run_form(solver, cfg, warmup, steps, keep_states=False):
         if keep_states is not None:
             st = solver.state()
             states.append((st["u"].copy(), st["dp"].copy()))
-    solver.spmv_timing(True)  # HIP events around every SpMV of the timed steps
+    solver.spmv_timing(live_every)  # HIP events around every k-th SpMV of the timed steps
     t0 = time.perf_counter()
     for k in range(steps):
         info = solver.step(cfg)
@@ -75,7 +96,7 @@ def run_form(solver, cfg, warmup, steps, keep_states=False):
             # state readback happens after the step returned; excluded below
             pass
     wall = time.perf_counter() - t0
-    spmv = solver.spmv_timing(False)
+    spmv = solver.spmv_timing(0)
     if keep_states is not None and steps:
         st = solver.state()
         states.append((st["u"].copy(), st["dp"].copy()))
@@ -299,12 +320,13 @@ def main():
 
     # SpMV kernel time inside the timed steps (HIP events on the library's
     # stream around every solver SpMV), per value precision
-    b64, b32 = spmv_bytes(info, 8), spmv_bytes(info, 4)
+    b64, b32 = spmv_bytes(info, 8, fused_dot=True), spmv_bytes(info, 4, fused_dot=True)
     ms64 = live64["fp64_ms"] / max(live64["fp64_n"], 1)
     ms32 = live32["fp32_ms"] / max(live32["fp32_n"], 1)
     gbps64, gbps32 = b64 / ms64 / 1e6, b32 / ms32 / 1e6
-    log(f"SpMV in the timed steps: fp64 {ms64:.3f} ms x {live64['fp64_n']} ({gbps64:.0f} GB/s), "
-        f"fp32 {ms32:.3f} ms x {live32['fp32_n']} ({gbps32:.0f} GB/s)")
+    log(f"SpMV in the timed steps (every 16th launch timed): fp64 {ms64:.3f} ms x "
+        f"{live64['fp64_n']} ({gbps64:.0f} GB/s), fp32 {ms32:.3f} ms x {live32['fp32_n']} "
+        f"({gbps32:.0f} GB/s)")
     iso64 = solver.bench_spmv(False, args.spmv_reps)
     iso32 = solver.bench_spmv(True, args.spmv_reps)
     log(f"SpMV back to back: fp64 {iso64:.3f} ms, fp32 {iso32:.3f} ms")
@@ -359,9 +381,13 @@ def main():
                  "back_to_back_fp64_ms": iso64, "back_to_back_fp32_ms": iso32,
                  "tuned_shapes": shapes,
                  "csr_equivalent_fp64_GBps": csr_bytes(info) / ms64 / 1e6},
-        "roofline": {"bound": "hbm", "kernel": "k_spmv<float> (FP32 node-block SpMV)",
+        "roofline": {"bound": "hbm",
+                     "kernel": f"k_spmv<float, {shape_lanes[shapes['fp32']['best']]}, "
+                               f"{shape_unr[shapes['fp32']['best']]}, 1, false> "
+                               "(FP32 node-block SpMV with the fused q.t dot)",
                      "achieved": gbps32, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": gbps32 / HBM_PEAK_GBPS, "traffic": None},
+                     "frac": gbps32 / HBM_PEAK_GBPS,
+                     "traffic": measured_traffic("fp32", shapes["fp32"]["best"], info)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -231,11 +231,12 @@ int pm_bench_spmv(pm_ctx *ctx, int use32, int reps, double *ms_per_launch);
 int pm_tune_spmv(pm_ctx *ctx, int use32, int reps, int *best, double *ms_per_shape);
 /* Selects SpMV shape `shape` (0 .. PM_SPMV_SHAPES-1) without timing. */
 int pm_set_spmv_shape(pm_ctx *ctx, int use32, int shape);
-/* Live SpMV timing: with on != 0 every SpMV the single-system solvers launch
- * is bracketed by HIP events on the library's stream (launches skipped after
- * convergence are not counted).  stats (may be NULL) receives what was
- * accumulated since the previous call -- {ms FP64, launches FP64, ms FP32,
- * launches FP32} -- and the accumulators restart from zero. */
+/* Live SpMV timing: with on = k > 0 every k-th SpMV the single-system
+ * solvers launch is bracketed by HIP events on the library's stream (launches
+ * skipped after convergence are not counted; on = 0 switches it off).  stats
+ * (may be NULL) receives what was accumulated since the previous call --
+ * {ms FP64, launches timed FP64, ms FP32, launches timed FP32} -- and the
+ * accumulators restart from zero. */
 int pm_spmv_timing(pm_ctx *ctx, int on, double *stats);
 /* Average device time of a plain copy kernel over `bytes` bytes (reads
  * bytes, writes bytes): the reference point for HBM traffic counters. */

@@ -734,7 +734,8 @@ extern "C" int pm_spmv_timing(pm_ctx *c, int on, double *stats) {
   }
   c->spmv_ms[0] = c->spmv_ms[1] = 0.0;
   c->spmv_n[0] = c->spmv_n[1] = 0;
-  c->time_spmv = on != 0;
+  c->time_spmv = on > 0 ? on : 0;
+  c->spmv_seq = 0;
   return PM_OK;
 }
 
@@ -766,8 +767,9 @@ extern "C" int pm_bench_spmv(pm_ctx *c, int use32, int reps, double *ms_per_laun
   return PM_OK;
 }
 
-// The SpMV shape tuner: times every (LANES, UNR) shape on the current matrix
-// and keeps the fastest for that value precision.  Shapes sum in different
+// The SpMV shape tuner: times every (LANES, UNR) shape on the current matrix,
+// in the form the Krylov loops launch it (fused dot and finisher), and keeps
+// the fastest for that value precision.  Shapes sum in different
 // orders, so results move at the rounding level when the shape changes; the
 // choice stays fixed until the next call.
 extern "C" int pm_tune_spmv(pm_ctx *c, int use32, int reps, int *best, double *ms_per_shape) {
@@ -778,6 +780,9 @@ extern "C" int pm_tune_spmv(pm_ctx *c, int use32, int reps, int *best, double *m
   PM_TRY(pm_ensure_workspace(c, c->A.neq));
   double *x = c->d_wk[0], *y = c->d_wk[1];
   PM_CHECK_HIP(hipMemsetAsync(x, 0, sizeof(double) * c->A.neq, c->stream));
+  PmScalars z;
+  memset(&z, 0, sizeof(z));  // the fused form skips its work once done is set
+  PM_CHECK_HIP(hipMemcpyAsync(c->d_scal, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
   const int p = use32 ? 1 : 0, keep = c->spmv_cfg[p], ordered = c->ordered;
   c->ordered = 0;
   int arg = -1;
@@ -785,7 +790,7 @@ extern "C" int pm_tune_spmv(pm_ctx *c, int use32, int reps, int *best, double *m
   for (int k = 0; k < PM_SPMV_NCFG; k++) {
     c->spmv_cfg[p] = k;
     auto spmv = [&](const double *a, double *b) {
-      return c->batch ? pm_batch_spmv_dev(c, a, b, use32) : pm_spmv_dev(c, a, b, use32);
+      return c->batch ? pm_batch_spmv_dev(c, a, b, use32) : pm_spmv_fused_dev(c, a, b, use32);
     };
     int rc = spmv(x, y);  // warm-up
     if (rc == PM_OK) rc = hipEventRecord(c->ev0, c->stream) == hipSuccess ? PM_OK : PM_ERR_HIP;

@@ -173,7 +173,9 @@ struct pm_ctx {
   // single-system solver launches, harvested after each host sync; pairs
   // tagged with an iteration past convergence (launched but skipped by the
   // kernels' done check) are dropped
-  int time_spmv = 0;
+  int time_spmv = 0;                // 0 off, k: every k-th SpMV launch
+  long long spmv_seq = 0;
+  bool tev_open = false;
   int cur_iter = 0;                 // iteration tag of the next launches
   std::vector<hipEvent_t> tev;      // 2 per pending pair
   std::vector<int> tev_kind, tev_iter;
@@ -200,6 +202,7 @@ int pm_ensure_workspace(pm_ctx *c, int n);
 
 // kernels entry points (pm_blas.hip, pm_krylov.hip, pm_element.hip, pm_march.hip)
 int pm_spmv_dev(pm_ctx *c, const double *x, double *y, int use32);
+int pm_spmv_fused_dev(pm_ctx *c, const double *x, double *y, int use32);
 int pm_solve_dev(pm_ctx *c, const pm_solver_cfg *cfg, const double *b, double *x,
                  pm_solve_info *info);
 int pm_pform_dev(pm_ctx *c, int isw, int lhs, int rhs, double dt, double t, double *bvec);

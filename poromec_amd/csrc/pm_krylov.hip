@@ -202,7 +202,8 @@ struct SpmvDots {  // the dots fused into the fast SpMV: y.w and y.y
 };
 
 static void spmv_time_begin(pm_ctx *c) {
-  if (!c->time_spmv) return;
+  c->tev_open = false;
+  if (!c->time_spmv || (c->spmv_seq++ % c->time_spmv) != 0) return;  // every time_spmv-th
   if ((size_t)(2 * c->tev_used + 2) > c->tev.size()) {
     hipEvent_t a = nullptr, b = nullptr;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
@@ -215,10 +216,12 @@ static void spmv_time_begin(pm_ctx *c) {
     c->tev_iter.push_back(0);
   }
   (void)hipEventRecord(c->tev[2 * c->tev_used], c->stream);
+  c->tev_open = true;
 }
 
 static void spmv_time_end(pm_ctx *c, int use32) {
-  if (!c->time_spmv || (size_t)(2 * c->tev_used + 2) > c->tev.size()) return;
+  if (!c->tev_open) return;
+  c->tev_open = false;
   (void)hipEventRecord(c->tev[2 * c->tev_used + 1], c->stream);
   c->tev_kind[c->tev_used] = use32 ? 1 : 0;
   c->tev_iter[c->tev_used] = c->cur_iter;
@@ -271,6 +274,14 @@ static void launch_spmv(pm_ctx *c, int use32, const double *x, double *y, const 
 
 int pm_spmv_dev(pm_ctx *c, const double *x, double *y, int use32) {
   launch_spmv<0, false>(c, use32, x, y, nullptr, nullptr, nullptr, NoEpi{});
+  PM_CHECK_HIP(hipGetLastError());
+  return PM_OK;
+}
+
+// the product as the Krylov loops launch it: y = A x with the dot y.x fused
+// and reduced by a finisher (what the shape tuner times)
+int pm_spmv_fused_dev(pm_ctx *c, const double *x, double *y, int use32) {
+  launch_spmv<1, false>(c, use32, x, y, x, nullptr, c->d_scal, NoEpi{});
   PM_CHECK_HIP(hipGetLastError());
   return PM_OK;
 }

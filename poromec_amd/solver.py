@@ -237,11 +237,12 @@ class MefSolver:
     def set_spmv_shape(self, use32: bool, shape: int):
         L.check(self.lib.pm_set_spmv_shape(self.ctx, int(use32), int(shape)), "pm_set_spmv_shape")
 
-    def spmv_timing(self, on: bool) -> dict:
-        """Switch live SpMV timing on or off; returns what was accumulated
-        since the previous call (ms and launches per value precision)."""
+    def spmv_timing(self, every: int) -> dict:
+        """Time every `every`-th solver SpMV with HIP events (0: off); returns
+        what was accumulated since the previous call (ms and timed launches
+        per value precision)."""
         st = np.zeros(4)
-        L.check(self.lib.pm_spmv_timing(self.ctx, int(on), L.ptr(st)), "pm_spmv_timing")
+        L.check(self.lib.pm_spmv_timing(self.ctx, int(every), L.ptr(st)), "pm_spmv_timing")
         return dict(fp64_ms=st[0], fp64_n=int(st[1]), fp32_ms=st[2], fp32_n=int(st[3]))
 
     def bench_copy(self, nbytes: int, reps: int = 20) -> float:
