@@ -172,9 +172,28 @@ def batch_form(args):
         out[name] = {"box_steps_per_s": args.nbox * args.steps / wall, "ms_per_step": 1e3 * wall / max(args.steps, 1),
                      "iters_per_box_step": {"min": int(it.min()), "mean": float(it.mean()), "max": int(it.max())}}
         log(f"  {name}: {out[name]}")
+    s.close()
+    # the same boxes one at a time (a sample of them), for comparison
+    seq = None
+    if args.seq_sample > 0:
+        k = min(args.seq_sample, args.nbox)
+        wall = 0.0
+        for d in decks[:k]:
+            one = MefSolver(d)
+            cfg = SolverConfig(tol=1e-10)
+            for _ in range(args.warmup):
+                one.step(cfg)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                one.step(cfg)
+            wall += time.perf_counter() - t0
+            one.close()
+        seq = {"boxes_timed": k, "box_steps_per_s": k * args.steps / wall}
+        log(f"  one box at a time (fp64, first {k} boxes): {seq}")
     print(json.dumps({"form": "batch", "nbox": args.nbox, "n_range": [args.nmin, args.nmax],
                       "neq": s.neq, "steps": args.steps, "warmup": args.warmup,
-                      "setup_s": setup_s, "spmv_shapes": shapes, **out}), flush=True)
+                      "setup_s": setup_s, "spmv_shapes": shapes, **out,
+                      "one_box_at_a_time_fp64": seq}), flush=True)
 
 
 def adjoint_form(args):
@@ -241,6 +260,8 @@ def main():
     ap.add_argument("--nbox", type=int, default=256)
     ap.add_argument("--nmin", type=int, default=12)
     ap.add_argument("--nmax", type=int, default=32)
+    ap.add_argument("--seq-sample", type=int, default=8,
+                    help="batch form: boxes also solved one at a time, for comparison")
     ap.add_argument("--nsteps", type=int, default=50, help="adjoint form: march length")
     ap.add_argument("--nsens", type=int, default=64)
     ap.add_argument("--ckpt", type=int, default=10)

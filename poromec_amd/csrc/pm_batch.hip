@@ -67,7 +67,7 @@ struct CountDone {  // epilogue wrapper: counts boxes that finish
 };
 
 template <typename VT, int LANES, int UNR, int NV>
-__global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, const int *__restrict__ fr,
+__global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, int neq, const int *__restrict__ fr,
                                                   const int *__restrict__ nr,
                                                   const int64_t *__restrict__ colptr,
                                                   const int64_t *__restrict__ valptr,
@@ -81,6 +81,7 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, const int *__rest
   const PmScalars *Sb = S + box;
   if (Sb->done) return;
   constexpr int GROUPS = PM_WG / LANES;
+  const __amdgpu_buffer_rsrc_t xr = pm_rsrc(x, (unsigned int)neq * 8u);
   const int gb = B.wg0[box], gc = B.wg0[box + 1] - gb;
   const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
   const int64_t nb0 = B.blk0[box], nblk = B.blk0[box + 1] - nb0;
@@ -94,7 +95,10 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, const int *__rest
     nx = pm_block_meta(k + GROUPS, nb0 + k1, colptr, valptr, nr);
     const int n = cur.n;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    pm_spmv_block<VT, LANES, UNR>(lane, cur, ncol, val, x, a);
+    if constexpr (LANES == 64 && sizeof(VT) == 4)
+      pm_spmv_block64<VT, UNR>(lane, cur, ncol, val, xr, a);
+    else
+      pm_spmv_block<VT, LANES, UNR>(lane, cur, ncol, val, x, a);
     if (lane == LANES - 1) {
       const int f = fr[k];
       for (int q = 0; q < n; q++) {
@@ -180,7 +184,8 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
 #define PM_BSPMV_CASE(id, L, U)                                                                 \
   case id:                                                                                      \
     hipLaunchKernelGGL((k_spmv_b<VT, L, U, NV>), dim3(B->G), dim3(PM_WG), 0, c->stream,        \
-                       batch_args(B), A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w,    \
+                       batch_args(B), A.neq, A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x,   \
+                       y, w,                                                                    \
                        B->d_S);                                                                 \
     break;
   switch (cfg) {

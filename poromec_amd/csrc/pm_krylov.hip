@@ -28,13 +28,14 @@
 // ---------------------------------------------------------------------------
 template <typename VT, int LANES, int UNR, int NV, bool SCALE_M>
 __global__ __launch_bounds__(PM_WG) void k_spmv(
-    int nrn, const int *__restrict__ fr, const int *__restrict__ nr,
+    int nrn, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
     const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
     const int *__restrict__ ncol, const VT *__restrict__ val, const double *__restrict__ x,
     double *__restrict__ y, const double *__restrict__ w, const double *__restrict__ m,
     double *partials, const PmScalars *S) {
   if (S != nullptr && S->done) return;
   constexpr int GROUPS = PM_WG / LANES;
+  const __amdgpu_buffer_rsrc_t xr = pm_rsrc(x, (unsigned int)neq * 8u);
   const int lane = threadIdx.x % LANES, grp = threadIdx.x / LANES;
   const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
   int64_t k0, k1;
@@ -57,7 +58,10 @@ __global__ __launch_bounds__(PM_WG) void k_spmv(
       }
     }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    pm_spmv_block<VT, LANES, UNR>(lane, cur, ncol, val, x, a);
+    if constexpr (LANES == 64 && sizeof(VT) == 4)
+      pm_spmv_block64<VT, UNR>(lane, cur, ncol, val, xr, a);
+    else
+      pm_spmv_block<VT, LANES, UNR>(lane, cur, ncol, val, x, a);
     if (lane == LANES - 1) {
       for (int q = 0; q < n; q++) {
         double yq = a[q];
@@ -111,7 +115,7 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
 #define PM_SPMV_CASE(id, L, U)                                                                  \
   case id:                                                                                      \
     hipLaunchKernelGGL((k_spmv<VT, L, U, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream,   \
-                       A.nrn, A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w, m,          \
+                       A.nrn, A.neq, A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w, m,   \
                        c->d_partials, S);                                                       \
     break;
   switch (cfg) {

@@ -104,12 +104,43 @@ def _gravity_box():
     return d
 
 
-@pytest.mark.parametrize("which", ["column", "gravity_box"])
+def _face_load_box():
+    """Gravity box with all three face-load types: a type-41 normal pressure
+    and a type-43 hydraulic flux on the top faces, a type-40 traction on the
+    faces of one side."""
+    d = _gravity_box()
+    load, fload = P.empty_load_table()
+    P.set_traction_load(load, fload, 1, (1e-4, -2e-4, -1e-3))
+    load[1] = (41, 2, 2)
+    fload[1, 0, :2] = (0.0, 1e30)
+    fload[1, 1, :2] = (5e-4, 5e-4)
+    load[2] = (43, 2, 2)
+    fload[2, 0, :2] = (0.0, 1e30)
+    fload[2, 1, :2] = (1e-6, 1e-6)
+    d.load, d.fload = load, fload
+    ne = d.numel
+    top = np.arange(ne) >= d.nx * d.ny * (d.nz - 1)
+    side = (np.arange(ne) % d.nx) == 0
+    d.eload = np.zeros((ne, 7), np.int32)
+    d.eload[top, 0] = 2
+    d.eload[side, 4] = 1
+    d.eloadp = np.zeros((ne, 7), np.int32)
+    d.eloadp[top, 0] = 3
+    return d
+
+
+def _deck_of(which):
+    if which == "column":
+        return P.column_deck(2, 2, 6, dt=0.01, nsteps=6)
+    return _gravity_box() if which == "gravity_box" else _face_load_box()
+
+
+@pytest.mark.parametrize("which", ["column", "gravity_box", "face_load_box"])
 def test_ordered_assembly_bitwise(which):
     """isw 2 matrix and right-hand side, and isw 4 residual (face loads,
     gravity), summed in element order: equal to the reference's bit for bit."""
     from poromec_amd.solver import MefSolver
-    d = P.column_deck(2, 2, 6, dt=0.01, nsteps=6) if which == "column" else _gravity_box()
+    d = _deck_of(which)
     st = O.build_structure(d)
     s = MefSolver(d)
     s.set_ordered(True)
@@ -138,13 +169,13 @@ def test_ordered_assembly_bitwise(which):
     s.close()
 
 
-@pytest.mark.parametrize("which", ["column", "gravity_box"])
+@pytest.mark.parametrize("which", ["column", "gravity_box", "face_load_box"])
 def test_ordered_march_bitwise(which):
     """The whole backward-Euler march in the reference's order: the oracle's
     nonlinear and Krylov iteration counts exactly, and u, dp and porosity
     bit for bit after every step."""
     from poromec_amd.solver import MefSolver, SolverConfig
-    d = P.column_deck(2, 2, 8, dt=0.01, nsteps=6) if which == "column" else _gravity_box()
+    d = P.column_deck(2, 2, 8, dt=0.01, nsteps=6) if which == "column" else _deck_of(which)
     st = O.build_structure(d)
     m = O.OracleModel(d, st)
     s = MefSolver(d)
