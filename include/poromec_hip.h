@@ -207,7 +207,8 @@ int pm_batch_graphs(pm_ctx *ctx, int on);
  * dJ/dlog(e(3)) (permeability) in grad_logk, one entry per material (numat;
  * either may be NULL).  Linear elastic materials, one solve per step
  * (tight solver tolerance); forward states are recomputed from checkpoints
- * taken every ckpt_every steps.  Not available in the batching form. */
+ * taken every ckpt_every steps.  Not available in the batching form.  On an
+ * error return the nodal state is that of some step of the march. */
 int pm_adjoint(pm_ctx *ctx, int nsteps, double dt, const pm_solver_cfg *cfg, int nsens,
                const int *sensor_dof, const double *target, int ckpt_every, double *J,
                double *grad_logE, double *grad_logk);

@@ -644,6 +644,10 @@ static int ensure_ordered(pm_ctx *c) {
   PM_TRY(up(&c->d_orsrc, o.rsrc));
   PM_TRY(up(&c->d_opptr, o.pptr));
   PM_TRY(up(&c->d_opsrc, o.psrc));
+  for (double **d : {&c->d_ke, &c->d_pe, &c->d_pv}) {
+    if (*d) (void)hipFree(*d);
+    *d = nullptr;
+  }
   PM_CHECK_HIP(hipMalloc((void **)&c->d_ke, sizeof(double) * numel * PM_NST * PM_NST));
   PM_CHECK_HIP(hipMalloc((void **)&c->d_pe, sizeof(double) * numel * PM_NST));
   PM_CHECK_HIP(hipMalloc((void **)&c->d_pv, sizeof(double) * numel * 8));
