@@ -65,6 +65,11 @@ typedef struct {
   int maxnlit;     /* nonlinear iterations per step (nl macro, default 500) */
   double nltol;    /* nonlinear tolerance (nl macro, default 1e-4) */
   int stop_crit;   /* cal_residuo_pm criterion: 1 or 2 */
+  int maxnlit_accept; /* 0: reaching maxnlit is PM_ERR_NLIT, as stop_mef at
+                         Main.f:1046-1053; nonzero: the step ends there and is
+                         kept (the goto 420 that code keeps commented out, the
+                         behaviour the decks shipped with the reference's
+                         outputs were run with) */
 } pm_solver_cfg;
 
 typedef struct {

@@ -48,7 +48,8 @@ class _Model(C.Structure):
 class _Cfg(C.Structure):
     _fields_ = [
         ("solver", C.c_int), ("precond", C.c_int), ("maxit", C.c_int), ("ngram", C.c_int),
-        ("maxnlit", C.c_int), ("stop_crit", C.c_int), ("solvtol", C.c_double),
+        ("maxnlit", C.c_int), ("stop_crit", C.c_int), ("maxnlit_accept", C.c_int),
+        ("solvtol", C.c_double),
         ("tol", C.c_double),
     ]
 
@@ -176,7 +177,7 @@ class OracleModel:
     """All state of the oracle's time march for one deck."""
 
     def __init__(self, deck, st=None, solver=11, precond=2, solvtol=1e-10, maxit=50000,
-                 ngram=150, maxnlit=500, tol=1e-4, stop_crit=1):
+                 ngram=150, maxnlit=500, tol=1e-4, stop_crit=1, maxnlit_accept=False):
         self.deck = deck
         self.st = st if st is not None else build_structure(deck)
         st = self.st
@@ -199,7 +200,8 @@ class OracleModel:
         self.x0 = np.zeros(neq)
         self.m = np.zeros(neq)
         self.env = make_env(deck.load, deck.fload, deck.dt, 0.0, deck.gravity, deck.gravity_mod)
-        self.cfg = _Cfg(solver, precond, maxit, ngram, maxnlit, stop_crit, solvtol, tol)
+        self.cfg = _Cfg(solver, precond, maxit, ngram, maxnlit, stop_crit, int(maxnlit_accept),
+                        solvtol, tol)
         self._md = _Model()
         md = self._md
         md.numel, md.nnode, md.nnodev, md.neq, md.nad = (st["numel"], nnode, deck.nnodev, neq,

@@ -228,7 +228,7 @@ int orc_residuo(const orc_model *md, const double *b, int nli, double tol, doubl
 /* One `solv` macro step (Main.f:876-1123) with stop criterion 1 or 2 and
  * the default newton_raphson = false.  env->t must hold t_n; it is advanced here.
  * Returns 0, or a negative code on solver / element failure, or -10 when
- * maxnlit is reached. */
+ * maxnlit is reached (unless cfg->maxnlit_accept, which ends the step there). */
 int orc_step(orc_model *md, const orc_cfg *cfg, orc_env *env, orc_step_info *info) {
   memset(info, 0, sizeof(*info));
   env->t = env->t + env->dt;
@@ -257,7 +257,10 @@ int orc_step(orc_model *md, const orc_cfg *cfg, orc_env *env, orc_step_info *inf
     info->nsolve++;
     if (it < 0) return it;
     orc_delta_update(md);
-    if (i >= cfg->maxnlit) return -10;
+    if (i >= cfg->maxnlit) {
+      if (cfg->maxnlit_accept) break;
+      return -10;
+    }
     i++;
     lhs = 0;
   }

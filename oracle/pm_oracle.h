@@ -65,6 +65,8 @@ typedef struct {
   int solver;   /* 1 pcg, 2 gmres, 4 bicgstab, 7 minres, 11 rpsqmr */
   int precond;  /* 2 diag, 5 |diag| */
   int maxit, ngram, maxnlit, stop_crit;
+  int maxnlit_accept; /* nonzero: reaching maxnlit ends the step (the goto 420
+                         that Main.f:1046-1053 keeps commented out) */
   double solvtol, tol;
 } orc_cfg;
 

@@ -24,7 +24,7 @@ class SolverCfg(C.Structure):
         ("solver", C.c_int), ("precond", C.c_int), ("maxit", C.c_int), ("nkrylov", C.c_int),
         ("tol", C.c_double), ("precision", C.c_int), ("ir_inner_tol", C.c_double),
         ("ir_maxouter", C.c_int), ("maxnlit", C.c_int), ("nltol", C.c_double),
-        ("stop_crit", C.c_int),
+        ("stop_crit", C.c_int), ("maxnlit_accept", C.c_int),
     ]
 
 

@@ -306,7 +306,7 @@ int pm_step_dev(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_info *in
         M = march_args(c, dt);
         hipLaunchKernelGGL(k_delta_update, dim3(gn), dim3(tb), 0, c->stream, M);
         if (i >= cfg->maxnlit) {
-          rc = PM_ERR_NLIT;
+          if (!cfg->maxnlit_accept) rc = PM_ERR_NLIT;
           break;
         }
         i++;
@@ -347,7 +347,7 @@ int pm_step_dev(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_info *in
       M = march_args(c, dt);
       hipLaunchKernelGGL(k_delta_update, dim3(gn), dim3(tb), 0, c->stream, M);
       if (i >= cfg->maxnlit) {
-        rc = PM_ERR_NLIT;
+        if (!cfg->maxnlit_accept) rc = PM_ERR_NLIT;
         break;
       }
       i++;

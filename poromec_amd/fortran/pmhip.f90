@@ -26,6 +26,7 @@ module pmhip
     integer(c_int) :: maxnlit = 500
     real(c_double) :: nltol = 1.0d-4
     integer(c_int) :: stop_crit = 2
+    integer(c_int) :: maxnlit_accept = 0
   end type
 
   type, bind(C) :: pm_solve_info

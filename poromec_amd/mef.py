@@ -1,0 +1,174 @@
+"""Runs a reference input deck on the MI355X and writes the reference's
+output files: the ``mef`` program (``Main.f``) for the macros that
+:mod:`poromec_amd.deck` reads.
+
+    python -m poromec_amd.mef <deck> <output prefix> [--fp32ir] [--ordered]
+
+mirrors ``mef <input> <output>`` (``Main.f:336-402``: the input file, then
+the prefix of every output file).  Files written, named as ``name``
+(``Filenames.f``) names them:
+
+* ``<prefix>_up_node_<n>.txt`` for every node of ``setpnode``, one line per
+  ``pnup``: ``istep, t, u, v, w, p`` in ``(i9,10e18.10)`` under the header
+  line of ``printnode`` (``Pdata.f:733-760``);
+* ``<prefix>_log_1.txt``: the solver log, one line per linear solve;
+* ``<prefix>_log_nonlinear_1.txt``: ``step istep t`` per ``solv`` macro
+  (``Main.f:892``) and the nonlinear iterations it took;
+* ``<prefix>_t_1.txt``: device times and the sizes of the system.
+
+Every step runs in the HIP library (``pm_step``); there is no CPU path.
+"""
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import sys
+import time
+
+import numpy as np
+
+from .deck import MefInput, read_deck
+from .problem import NDF
+from .solver import SOLVER_NAMES, SOLVERS, MefSolver
+
+
+def fortran_e(v: float, w: int = 18, d: int = 10) -> str:
+    """``v`` as the Fortran edit descriptor Ew.d writes it: ``0.ddddE+xx``,
+    the exponent without its letter when it needs three digits."""
+    v = float(v)
+    if v != v or v in (float("inf"), float("-inf")):
+        s = "NaN" if v != v else ("Infinity" if v > 0 else "-Infinity")
+        return s.rjust(w)
+    if v == 0.0:
+        body, sign = "0." + "0" * d + "E+00", "-" if np.signbit(v) else ""
+    else:
+        sign = "-" if v < 0 else ""
+        m, e = f"{abs(v):.{d - 1}e}".split("e")
+        exp = int(e) + 1
+        digits = m.replace(".", "")
+        if abs(exp) <= 99:
+            tail = f"E{'+' if exp >= 0 else '-'}{abs(exp):02d}"
+        else:
+            tail = f"{'+' if exp >= 0 else '-'}{abs(exp):03d}"
+        body = "0." + digits + tail
+    s = sign + body
+    return s.rjust(w) if len(s) <= w else "*" * w
+
+
+def printnode_line(istep: int, t: float, u) -> str:
+    """One line of a ``printnode`` history: ``(i9,10e18.10)``."""
+    return f"{istep:9d}" + fortran_e(t) + "".join(fortran_e(x) for x in u)
+
+
+def printnode_header(node: int, name: str = "DeslocAndPress") -> str:
+    return f"# Variacao {name} no tempo no ponto{node:9d}"
+
+
+def read_printnode(path):
+    """(istep, t, values) arrays of a printnode history file."""
+    rows = [ln.split() for ln in open(path) if ln.strip() and not ln.startswith("#")]
+    a = np.array([[float(v.replace("D", "E")) for v in r] for r in rows])
+    return a[:, 0].astype(int), a[:, 1], a[:, 2:]
+
+
+def run(inp: MefInput, prefix: str, precision: str = "fp64", ordered: bool = False,
+        device: int = 0, echo=None) -> dict:
+    """Runs the deck's macro program; returns the histories of the print
+    nodes ({node: (nsteps+, 6) array of istep, t, u, v, w, p}) and per-step
+    solver statistics."""
+    wall0 = time.perf_counter()
+    deck = inp.deck
+    cfg = dataclasses.replace(inp.cfg, precision=precision)
+    s = MefSolver(deck, device=device, reorder=inp.reorder)
+    try:
+        if ordered:
+            s.set_ordered(True)
+        nv = deck.nnodev
+        u0 = np.zeros((s.nnode, NDF))
+        u0[:nv] = inp.u0
+        s.set_state(u=u0, u0=u0, t=0.0)
+        for n in inp.pnodes:
+            if not 1 <= n <= s.nnode:
+                raise ValueError(f"print node {n} outside 1..{s.nnode}")
+        hist = {n: [] for n in inp.pnodes}
+        stats = []
+        dt, t, istep = 1.0, 0.0, 0
+        new_file = True
+        ms_elem = ms_solve = ms_total = 0.0
+        with open(f"{prefix}_log_1.txt", "w") as flog, \
+                open(f"{prefix}_log_nonlinear_1.txt", "w") as fnl:
+            flog.write("Solver control flop.\n")
+            fnl.write("Non-linear control flop.\n")
+            for op in inp.program:
+                if op[0] == "dt":
+                    dt = op[1]
+                elif op[0] == "pnup":
+                    if not inp.pnodes:
+                        raise ValueError("No node to print for PNUP!")
+                    u = s.state()["u"]
+                    for n in inp.pnodes:
+                        mode = "w" if new_file else "a"
+                        with open(f"{prefix}_up_node_{n}.txt", mode) as f:
+                            if new_file:
+                                f.write(printnode_header(n) + "\n")
+                            f.write(printnode_line(istep, t, u[n - 1]) + "\n")
+                        hist[n].append([istep, t, *u[n - 1]])
+                    new_file = False
+                elif op[0] == "solv":
+                    istep += 1
+                    t = t + dt
+                    fnl.write(f"step{istep:9d}{fortran_e(t, 13, 6)}\n")
+                    info = s.step(cfg, dt)
+                    ms_elem += info["ms_element"]
+                    ms_solve += info["ms_solve"]
+                    ms_total += info["ms_total"]
+                    for k in range(min(info["nsolve"], 16)):
+                        name = SOLVER_NAMES[info["used"][k]] if info["used"][k] in SOLVER_NAMES \
+                            else SOLVER_NAMES[SOLVERS[cfg.solver]]
+                        flog.write(f"{name.upper()}:  it{info['iters'][k]:10d} true rel resid "
+                                   f"{fortran_e(info['true_rel'][k], 17, 10)} tol "
+                                   f"{fortran_e(cfg.tol, 17, 10)}\n")
+                    fnl.write(f"nlit{info['nlit']:7d}\n")
+                    stats.append(dict(nlit=info["nlit"], iters=info["iters"]))
+                    if echo:
+                        echo(f"step {istep:6d} t {t:.6e} nlit {info['nlit']} "
+                             f"iters {info['iters']}")
+        m = s.matrix_info()
+        with open(f"{prefix}_t_1.txt", "w") as f:
+            f.write("# Arquivo de log do poro mecanico (MI355X)\n")
+            f.write("Tempos (seg):\n")
+            for key, v in (("ELMT", ms_elem), ("SOLVR", ms_solve), ("STEPS", ms_total)):
+                f.write(f"{key:<12}{v / 1e3:12.6f}\n")
+            f.write(f"{'TOTAL':<12}{time.perf_counter() - wall0:12.6f}\n")
+            f.write("Malha e sistema linear:\n")
+            for key, v in (("neq", s.neq), ("nad", s.nad), ("nnz", m["nnz"]),
+                           ("nnode", s.nnode), ("numel", deck.numel)):
+                f.write(f"{key:<10}{v:12d}\n")
+        return dict(hist={n: np.array(v) for n, v in hist.items()}, stats=stats,
+                    nnode=s.nnode, neq=s.neq, nad=s.nad)
+    finally:
+        s.close()
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("deck")
+    ap.add_argument("prefix")
+    ap.add_argument("--fp32ir", action="store_true",
+                    help="FP32 matrix values inside FP64 iterative refinement")
+    ap.add_argument("--ordered", action="store_true",
+                    help="reference summation order (bitwise against the oracle; slow)")
+    ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--quiet", action="store_true")
+    a = ap.parse_args(argv)
+    inp = read_deck(a.deck)
+    if inp.skipped and not a.quiet:
+        print("macros without output here:", " ".join(sorted(set(inp.skipped))))
+    r = run(inp, a.prefix, precision="fp32ir" if a.fp32ir else "fp64", ordered=a.ordered,
+            device=a.device, echo=None if a.quiet else print)
+    print(f"done: {len(r['stats'])} steps, nnode {r['nnode']} neq {r['neq']} nad {r['nad']}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -41,11 +41,13 @@ class SolverConfig:
     maxnlit: int = 500
     nltol: float = 1.0e-4
     stop_crit: int = 1
+    maxnlit_accept: bool = False  # keep the step at maxnlit instead of PM_ERR_NLIT
 
     def c(self) -> L.SolverCfg:
         return L.SolverCfg(SOLVERS[self.solver], self.precond, self.maxit, self.ngram, self.tol,
                            0 if self.precision == "fp64" else 1, self.ir_inner_tol,
-                           self.ir_maxouter, self.maxnlit, self.nltol, self.stop_crit)
+                           self.ir_maxouter, self.maxnlit, self.nltol, self.stop_crit,
+                           int(self.maxnlit_accept))
 
 
 def mesh_numbering(deck: Deck, reorder: bool = True):

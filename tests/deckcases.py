@@ -1,0 +1,70 @@
+"""The reference's example decks (tests/golden/decks, written by
+tests/golden/make_deck_fixtures.py) and the oracle's run of a deck's macro
+program, for the deck tests."""
+from __future__ import annotations
+
+import tarfile
+from pathlib import Path
+
+import numpy as np
+
+from oracle import oracle as O
+from poromec_amd.solver import SOLVERS
+
+GOLD = Path(__file__).resolve().parent / "golden" / "decks"
+CASES = ("exemplo1_mesh0_t2", "solo1", "solo_cilindro_hexa_1")
+
+
+def extract(case: str, dest: Path) -> Path:
+    """Unpacks a case's deck files into dest; returns the main deck's path."""
+    with tarfile.open(GOLD / f"{case}.tar.gz") as tar:
+        for m in tar.getmembers():
+            if not m.isfile() or "/" in m.name or m.name.startswith("."):
+                raise ValueError(f"unexpected member {m.name}")
+            (dest / m.name).write_bytes(tar.extractfile(m).read())
+    return dest / f"{case}.dat"
+
+
+def reference(case: str) -> dict:
+    """The reference's results for a case: its logged sizes and, per print
+    node, the up_node history (istep, t, u, v, w, p) of every step."""
+    z = np.load(GOLD / f"{case}_ref.npz")
+    out = {k: int(z[k]) for k in ("nnode", "neq", "nad", "numel")}
+    out["hist"] = {int(k[5:]): z[k] for k in z.files if k.startswith("node_")}
+    return out
+
+
+def oracle_run(inp, nsteps=None):
+    """The oracle's run of the deck's program: {node: (n, 6) history}, the
+    per-step iteration counts and the final nodal u."""
+    c = inp.cfg
+    deck = inp.deck
+    st = O.build_structure(deck, reorder=inp.reorder)
+    m = O.OracleModel(deck, st=st, solver=SOLVERS[c.solver], precond=c.precond, solvtol=c.tol,
+                      maxit=c.maxit, ngram=c.ngram, maxnlit=c.maxnlit, tol=c.nltol,
+                      stop_crit=c.stop_crit, maxnlit_accept=c.maxnlit_accept)
+    nv = deck.nnodev
+    m.u0[:nv] = inp.u0
+    m.u[:nv] = inp.u0
+    hist = {n: [] for n in inp.pnodes}
+    iters, istep, t = [], 0, 0.0
+    m.env.dt = 1.0
+    for op in inp.program:
+        if op[0] == "dt":
+            m.env.dt = op[1]
+        elif op[0] == "pnup":
+            for n in inp.pnodes:
+                hist[n].append([istep, t, *m.u[n - 1]])
+        elif op[0] == "solv":
+            if nsteps is not None and istep == nsteps:
+                break
+            istep += 1
+            t = t + m.env.dt
+            info = m.step()
+            iters.append(info["iters"])
+    return {n: np.array(v) for n, v in hist.items()}, iters, m.u.copy()
+
+
+def print_close(a, b, atol=1e-9, rtol=1e-9):
+    """Agreement within what a 10-decimal text print resolves."""
+    return np.abs(a - b) <= atol + rtol * np.abs(b)

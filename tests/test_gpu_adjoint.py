@@ -118,7 +118,7 @@ def test_autograd_wrapper():
     J = m(lE, lk)
     (2.0 * J).backward()
     J2, gE, gk = m._evaluate(lE.detach().numpy(), lk.detach().numpy())
-    assert float(J) == J2
+    assert float(J.detach()) == J2
     torch.testing.assert_close(lE.grad, torch.from_numpy(2.0 * gE))
     torch.testing.assert_close(lk.grad, torch.from_numpy(2.0 * gk))
     m.close()
