@@ -176,6 +176,16 @@ int pm_set_vectors(pm_ctx *ctx, const double *b0, const double *b, const double 
  * porosity_form, update_res_v2. */
 int pm_step(pm_ctx *ctx, double dt, const pm_solver_cfg *cfg, pm_step_info *info);
 
+/* Vertex stresses and Darcy flux of the current state, what the pnsf macro
+ * prints: tform_pm (Pform.f:433-700) with elmt17_pm isw 3
+ * (Elmlib03.f:2206-2256, darcy_flux Elmlib.f:3178-3200) and
+ * effective_stress (Pform.f:2487-2503).  tx, txe, txb: (6, nnode) total
+ * stress D eps - alpha dp, Terzaghi's tx + p and Biot's tx + alpha p,
+ * components xx yy zz xy yz xz; flux: (3, nnode) perm (rho_f g - grad p).
+ * Each vertex gets the average over its elements, summed in element order;
+ * other nodes get zero (txe: the pressure shift).  Any pointer may be NULL. */
+int pm_tform(pm_ctx *ctx, double *tx, double *txe, double *txb, double *flux);
+
 /* ---- reference summation order (no counterpart in the reference: it is the
  * reference's own arithmetic order) ----
  * on != 0: every dot product and norm of the solvers and of the residual

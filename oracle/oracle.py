@@ -103,6 +103,8 @@ def _load_lib():
     L.orc_step.restype = C.c_int
     L.orc_step.argtypes = [C.POINTER(_Model), C.POINTER(_Cfg), C.POINTER(OracleEnv),
                            C.POINTER(_StepInfo)]
+    L.orc_tform.restype = C.c_int
+    L.orc_tform.argtypes = [C.POINTER(_Model), _f64p, _f64p, _f64p, _f64p, C.POINTER(OracleEnv)]
     L.orc_tload.restype = C.c_double
     L.orc_tload.argtypes = [C.c_int, C.c_double, _f64p, C.c_double, _f64p,
                             C.POINTER(OracleEnv)]
@@ -221,6 +223,17 @@ class OracleModel:
         if rc:
             raise RuntimeError(f"orc_pform failed: {rc}")
         return bvec
+
+    def tform(self):
+        """Vertex stresses and flux of the current state (the pnsf macro):
+        total, Terzaghi (total + p), Biot (total + alpha p) as (nnode, 6) and
+        the Darcy flux (nnode, 3)."""
+        n = self.st["nnode"]
+        tx, txe, txb, flux = np.zeros((n, 6)), np.zeros((n, 6)), np.zeros((n, 6)), np.zeros((n, 3))
+        rc = lib.orc_tform(C.byref(self._md), tx, txe, txb, flux, C.byref(self.env))
+        if rc:
+            raise RuntimeError(f"orc_tform failed: {rc}")
+        return tx, txe, txb, flux
 
     def step(self):
         info = _StepInfo()

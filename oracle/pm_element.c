@@ -13,7 +13,7 @@
  *   rotmatrix    Elmlib.f:2397-2450     rotate       Elmlib.f:2518-2545
  *   face_normal_vector Elmlib.f:3678-3727, vet Matvec.f
  *   Gauss table  Elmlib.f:3837-3903 (15-digit constants)
- *   elmt17_pm    Elmlib03.f:1822-2636 (isw 2, 4, 7)
+ *   elmt17_pm    Elmlib03.f:1822-2636 (isw 2, 3, 4, 7), darcy_flux Elmlib.f:3178
  *   tload        Pload.f:220-377,  interpol Interpolation.f:1-40
  */
 #include <math.h>
@@ -334,9 +334,10 @@ double orc_tload(int nc, double t, const double *x, double u, double *vc, const 
 #define P(i) p[(i) - 1]
 #define U(i) u[(i) - 1]
 
-/* elmt17_pm (Elmlib03.f:1822-2636), isw = 2, 4, 7.
+/* elmt17_pm (Elmlib03.f:1822-2636), isw = 2, 3, 4, 7.
  * Returns 0, -1 on a non-positive Jacobian (jacob3d_m stops there), -2 on an
- * unsupported face-load type.  p must hold >= 68 values, s 68*68. */
+ * unsupported face-load type.  p must hold >= 68 values (120 for isw 3), s
+ * 68*68. */
 int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *x,
                const double *u, const double *dp, double *p, double *s, int isw,
                const orc_env *env) {
@@ -553,6 +554,51 @@ int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *x,
           }
         }
       }
+    }
+    return 0;
+  }
+
+  /* isw 3 (Elmlib03.f:2206-2256): at each vertex the total stress
+   * D eps - alpha dp (p(1:48)), Biot's effective stress, total + alpha p
+   * (p(49:96)), and the Darcy flux perm (rho_f g - grad p) (darcy_flux,
+   * Elmlib.f:3178-3200; p(97:120)) */
+  if (isw == 3) {
+    for (int i = 1; i <= 8; i++) {
+      int tp = 6 * i - 5, tp1 = 6 * i + 43, l = i + 60;
+      sfhexa8(hp, hpx, hpy, hpz, RN[i - 1], SN[i - 1], TN[i - 1], 0, 1);
+      if (jacob3d(hpx, hpy, hpz, xj, xji, x, &det, 8, 1)) return -1;
+      sfhexa20(hu, hux, huy, huz, RN[i - 1], SN[i - 1], TN[i - 1], 0, 1);
+      jacob3d(hux, huy, huz, xj, xji, x, &det, 20, 0);
+      deform3d(hux, huy, huz, u, epsi, 20);
+      stress3d(a, b, c, epsi, &P(tp));
+      double dpm = coef_biot * dp[i - 1];
+      double pm = coef_biot * U(l);
+      P(tp) = P(tp) - dpm;
+      P(tp + 1) = P(tp + 1) - dpm;
+      P(tp + 2) = P(tp + 2) - dpm;
+      P(tp1) = P(tp) + pm;
+      P(tp1 + 1) = P(tp + 1) + pm;
+      P(tp1 + 2) = P(tp + 2) + pm;
+      P(tp1 + 3) = P(tp + 3);
+      P(tp1 + 4) = P(tp + 4);
+      P(tp1 + 5) = P(tp + 5);
+    }
+    for (int i = 1; i <= 8; i++) {
+      int tp = 3 * i + 94;
+      sfhexa8(hp, hpx, hpy, hpz, RN[i - 1], SN[i - 1], TN[i - 1], 1, 1);
+      if (jacob3d(hpx, hpy, hpz, xj, xji, x, &det, 8, 1)) return -1;
+      double *q = &P(tp);
+      q[0] = fluid_d * gl[0];
+      q[1] = fluid_d * gl[1];
+      q[2] = fluid_d * gl[2];
+      for (int k = 0; k < 8; k++) {
+        q[0] = q[0] - hpx[k] * U(61 + k);
+        q[1] = q[1] - hpy[k] * U(61 + k);
+        q[2] = q[2] - hpz[k] * U(61 + k);
+      }
+      q[0] = perm * q[0];
+      q[1] = perm * q[1];
+      q[2] = perm * q[2];
     }
     return 0;
   }

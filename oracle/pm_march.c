@@ -7,6 +7,7 @@
  *   orc_pform          pform_pm sequential branch  Pform.f:1-429 (zeroing :163-168,
  *                      gather :307-381) + csr      Assbly.f:218-304
  *   orc_porosity       porosity_form               Pform.f:969-1176
+ *   orc_tform          tform_pm, effective_stress  Pform.f:433-700, 2487-2503
  *   orc_pload          pload_pm                    Pload.f:1-92
  *   orc_delta_predict  delta_predict_pm            Predict.f:117-158
  *   orc_delta_update   delta_update_pm             Predict.f:159-215
@@ -90,6 +91,57 @@ int orc_pform(orc_model *md, int isw, int lhs, int rhs, double *bvec, const orc_
     assemble_csr(sl, pl, ld, md->ia, md->ja, md->al, md->ad, bvec, md->neq, lhs, rhs);
   }
   free(sl);
+  return rc;
+}
+
+int orc_tform(orc_model *md, double *tx, double *txe, double *txb, double *flux,
+              const orc_env *env) {
+  int ld[NST];
+  double ul[NST], pl[NST], dpl[ORC_NENV], xl[3 * ORC_NENV];
+  double p[8 * 15];
+  int iqz[7] = {0, 0, 0, 0, 0, 0, 0};
+  int *ic = calloc((size_t)md->nnode, sizeof(int));
+  if (!ic) return -4;
+  memset(tx, 0, sizeof(double) * 6 * (size_t)md->nnode);
+  memset(txb, 0, sizeof(double) * 6 * (size_t)md->nnode);
+  memset(flux, 0, sizeof(double) * 3 * (size_t)md->nnode);
+  int rc = 0;
+  for (int nel = 0; nel < md->numel; nel++) {
+    gather(md, nel, ld, ul, dpl, xl, pl);
+    for (int i = 0; i < 8 * 15; i++) p[i] = 0.0;
+    int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
+    const double *el = md->e + (size_t)(ma - 1) * ORC_PROP;
+    rc = orc_elmt17(el, iqz, iqz, xl, ul, dpl, p, NULL, 3, env);
+    if (rc) break;
+    for (int i = 0; i < ORC_NENV; i++) {
+      int no = md->ix[(size_t)nel * NEN1 + i];
+      if (no <= 0) continue;
+      ic[no - 1]++;
+      for (int j = 0; j < 6; j++) {
+        tx[6 * (no - 1) + j] = tx[6 * (no - 1) + j] + p[6 * i + j];
+        txb[6 * (no - 1) + j] = txb[6 * (no - 1) + j] + p[6 * i + j + 48];
+      }
+      for (int j = 0; j < 3; j++) flux[3 * (no - 1) + j] = flux[3 * (no - 1) + j] + p[3 * i + j + 96];
+    }
+  }
+  for (int i = 0; i < md->nnode && rc == 0; i++) {
+    if (md->fnno[i] == 1) {
+      for (int j = 0; j < 6; j++) {
+        tx[6 * i + j] = tx[6 * i + j] / ic[i];
+        txb[6 * i + j] = txb[6 * i + j] / ic[i];
+      }
+      for (int j = 0; j < 3; j++) flux[3 * i + j] = flux[3 * i + j] / ic[i];
+    }
+    /* effective_stress: Terzaghi's effective stress tx + p on the diagonal */
+    double pn = md->u[NDF * i + NDF - 1];
+    txe[6 * i] = tx[6 * i] + pn;
+    txe[6 * i + 1] = tx[6 * i + 1] + pn;
+    txe[6 * i + 2] = tx[6 * i + 2] + pn;
+    txe[6 * i + 3] = tx[6 * i + 3];
+    txe[6 * i + 4] = tx[6 * i + 4];
+    txe[6 * i + 5] = tx[6 * i + 5];
+  }
+  free(ic);
   return rc;
 }
 

@@ -99,6 +99,11 @@ void orc_csrc_fill(const int *id, const int *num, const int64_t *ip, const int *
 /* ---- element loop + assembly (Pform.f, Assbly.f) ---- */
 int orc_pform(orc_model *md, int isw, int lhs, int rhs, double *bvec, const orc_env *env);
 void orc_porosity(orc_model *md, double *dporo, int *ic, const orc_env *env);
+/* tform_pm (Pform.f:433-700) + effective_stress (Pform.f:2487-2503): vertex
+ * averages of total stress tx (6 x nnode), Terzaghi tx + p (txe), Biot's
+ * tx + alpha p (txb) and the Darcy flux (3 x nnode); other nodes get zero. */
+int orc_tform(orc_model *md, double *tx, double *txe, double *txb, double *flux,
+              const orc_env *env);
 
 /* ---- solvers (Matvec.f, Iters.f, Precond.f, Solv.f) ---- */
 void orc_matvec_csrc(int neq, const int64_t *ia, const int *ja, const double *ad,

@@ -63,6 +63,9 @@ pm_ctx::~pm_ctx() {
   FREE(d_ke);
   FREE(d_pe);
   FREE(d_pv);
+  FREE(d_tf_ptr);
+  FREE(d_tf_src);
+  FREE(d_tf_out);
   if (batch) pm_batch_free(batch);
   FREE(d_eq_box);
   free_matrix(A);
@@ -328,6 +331,10 @@ extern "C" int pm_mesh_setup(pm_ctx *c, int numel, int nnodev, const int *ix, co
   PM_CHECK_HIP(hipSetDevice(c->device));
   PmHostMesh &m = c->mesh;
   m = PmHostMesh();
+  FREE(c->d_tf_ptr);  // per-mesh maps are rebuilt on first use
+  FREE(c->d_tf_src);
+  FREE(c->d_tf_out);
+  c->have_ordered_maps = false;
   m.numel = numel;
   m.nnodev = nnodev;
   PM_TRY(pm_host_promote(m, ix));
@@ -707,6 +714,12 @@ extern "C" int pm_step(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_i
   PM_CHECK_HIP(hipSetDevice(c->device));
   pm_step_info local;
   return pm_step_dev(c, dt, cfg, info ? info : &local);
+}
+
+extern "C" int pm_tform(pm_ctx *c, double *tx, double *txe, double *txb, double *flux) {
+  if (!c) return PM_ERR_ARG;
+  PM_CHECK_HIP(hipSetDevice(c->device));
+  return pm_tform_dev(c, tx, txe, txb, flux);
 }
 
 extern "C" int pm_set_ordered(pm_ctx *c, int on) {

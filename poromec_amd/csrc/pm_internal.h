@@ -146,6 +146,10 @@ struct pm_ctx {
   int64_t *d_opptr = nullptr, *d_opsrc = nullptr;
   double *d_ke = nullptr, *d_pe = nullptr, *d_pv = nullptr;  // element arrays
   bool have_ordered_maps = false;
+  // stress recovery: (element, corner) pairs per node and the outputs
+  int64_t *d_tf_ptr = nullptr;
+  int *d_tf_src = nullptr;
+  double *d_tf_out = nullptr;  // tx, txe, txb (6 x nnode each), flux (3 x nnode)
 
   // nodal loads
   int *d_nload = nullptr;   // (nnode, 4)
@@ -207,6 +211,7 @@ int pm_solve_dev(pm_ctx *c, const pm_solver_cfg *cfg, const double *b, double *x
                  pm_solve_info *info);
 int pm_pform_dev(pm_ctx *c, int isw, int lhs, int rhs, double dt, double t, double *bvec);
 int pm_porosity_dev(pm_ctx *c, double dt);
+int pm_tform_dev(pm_ctx *c, double *tx, double *txe, double *txb, double *flux);
 int pm_extract_diag(pm_ctx *c);
 int pm_make_val32(pm_ctx *c);
 int pm_step_dev(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_info *info);

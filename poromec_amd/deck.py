@@ -12,8 +12,8 @@ settings and the list of time-step and print macros; ``poromec_amd.mef``
 runs that program on the GPU and writes the reference's output files.
 
 Macros read (top level): ``config``, ``solver``, ``gravity``, ``nl``,
-``dt``, ``mesh``, ``setpnode``, ``pnup``, ``solv``, ``loop``/``next``,
-``stop``.  Mesh block: the parameter line, ``materials``, ``fmaterials``,
+``dt``, ``mesh``, ``setpnode``, ``pnup``, ``pnsf``, ``solv``,
+``loop``/``next``, ``stop``.  Mesh block: the parameter line, ``materials``, ``fmaterials``,
 ``hexa8``, ``coordinates``, ``constrainpmec``, ``elmtloads``,
 ``elmtpresloads``, ``loads`` (types 1-8 inline, 39-43 from a file),
 ``initialpres``, ``hydrostatic``, ``insert``/``return``, ``end``.
@@ -33,11 +33,11 @@ also read, each where the current reader would reject or misread it:
 * blocks the current reader does not know (``faceR1`` ... ``end faceR1``):
   their ``end <name>`` line closes that block and not the mesh block.
 
-Macros that only write post-processing files (``pgeo``, ``pnsf``, ``pres``,
-``setprint``, ``deltatc``, ...) are recorded in :attr:`MefInput.skipped` and
-otherwise ignored; macros that would change the system this path solves
-(initial stresses, nodal forces, hexa20 input, plasticity, ...) raise
-:class:`DeckError`.
+Macros that only write post-processing files this reader does not produce
+(``pgeo``, ``pres``, ``setprint``, ``deltatc``, ...) are recorded in
+:attr:`MefInput.skipped` and otherwise ignored; macros that would change
+the system this path solves (initial stresses, nodal forces, hexa20 input,
+plasticity, ...) raise :class:`DeckError`.
 """
 from __future__ import annotations
 
@@ -54,8 +54,8 @@ from .solver import SolverConfig
 # element type ids accepted for the hexa20/8 Biot element (elmt17_pm)
 ELEMENT_TYPES = (17, 7)
 # post-processing macros: they write files this path does not produce
-OUTPUT_MACROS = ("pgeo", "pnsf", "pres", "setprint", "setpi", "ppi", "pcoo", "pcolors",
-                 "pmecres", "ptermres", "deltatc")
+OUTPUT_MACROS = ("pgeo", "pres", "setprint", "setpi", "ppi", "pcoo", "pcolors", "pmecres",
+                 "ptermres", "deltatc")
 # macros that change the problem beyond this path
 UNSUPPORTED_TOP = ("block_pu", "conseq", "hextotet", "solvt", "solvm", "presolv", "neqs", "run")
 UNSUPPORTED_MESH = ("hexa20", "bar2", "tria3", "quad4", "tetra4", "tetra10", "constraindisp",
@@ -155,7 +155,7 @@ class MefInput:
     cfg: SolverConfig
     reorder: bool
     pnodes: list                   # nodes of setpnode (1-based)
-    program: list                  # ("solv",) / ("pnup",) / ("dt", value) in order
+    program: list                  # ("solv",) / ("pnup",) / ("pnsf",) / ("dt", value)
     skipped: list = field(default_factory=list)
     element_type: int = 17
     path: str = ""
@@ -188,7 +188,7 @@ def read_deck(path) -> MefInput:
                 continue
             program.extend(loop[1] * loop[0])
             loop = None
-        elif mc in ("solv", "pnup"):
+        elif mc in ("solv", "pnup", "pnsf"):
             ops.append((mc,))
         elif mc == "dt":
             if len(w) < 2:

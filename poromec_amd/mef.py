@@ -11,6 +11,11 @@ the prefix of every output file).  Files written, named as ``name``
 * ``<prefix>_up_node_<n>.txt`` for every node of ``setpnode``, one line per
   ``pnup``: ``istep, t, u, v, w, p`` in ``(i9,10e18.10)`` under the header
   line of ``printnode`` (``Pdata.f:733-760``);
+* for every ``pnsf``, the node histories of the total stress
+  ``_stress_node_<n>``, Terzaghi's effective stress ``_stressE_node_<n>``,
+  Biot's effective stress ``_stressB_node_<n>`` (xx yy zz xy yz xz), the
+  Darcy flux ``_flux_node_<n>`` and the porosity ``_poro_node_<n>``
+  (``Main.f:2310-2481``, ``Filenames.f`` codes 31-34, 37);
 * ``<prefix>_log_1.txt``: the solver log, one line per linear solve;
 * ``<prefix>_log_nonlinear_1.txt``: ``step istep t`` per ``solv`` macro
   (``Main.f:892``) and the nonlinear iterations it took;
@@ -64,6 +69,11 @@ def printnode_header(node: int, name: str = "DeslocAndPress") -> str:
     return f"# Variacao {name} no tempo no ponto{node:9d}"
 
 
+# pnsf outputs: file stem, printnode name (Main.f:2375-2481)
+PNSF_FILES = (("stress", "totalStress"), ("stressE", "eSstress"), ("stressB", "biotStress"),
+              ("flux", "darcyFlux"), ("poro", "poro"))
+
+
 def read_printnode(path):
     """(istep, t, values) arrays of a printnode history file."""
     rows = [ln.split() for ln in open(path) if ln.strip() and not ln.startswith("#")]
@@ -73,9 +83,10 @@ def read_printnode(path):
 
 def run(inp: MefInput, prefix: str, precision: str = "fp64", ordered: bool = False,
         device: int = 0, echo=None) -> dict:
-    """Runs the deck's macro program; returns the histories of the print
-    nodes ({node: (nsteps+, 6) array of istep, t, u, v, w, p}) and per-step
-    solver statistics."""
+    """Runs the deck's macro program; returns the pnup histories of the
+    print nodes ({node: array of rows istep, t, u, v, w, p}), the pnsf ones
+    ({"stress" | "stressE" | "stressB" | "flux" | "poro": {node: rows}}) and
+    per-step solver statistics."""
     wall0 = time.perf_counter()
     deck = inp.deck
     cfg = dataclasses.replace(inp.cfg, precision=precision)
@@ -91,9 +102,11 @@ def run(inp: MefInput, prefix: str, precision: str = "fp64", ordered: bool = Fal
             if not 1 <= n <= s.nnode:
                 raise ValueError(f"print node {n} outside 1..{s.nnode}")
         hist = {n: [] for n in inp.pnodes}
+        shist = {k: {n: [] for n in inp.pnodes} for k, _ in PNSF_FILES}
         stats = []
         dt, t, istep = 1.0, 0.0, 0
         new_file = True
+        new_sf = True
         ms_elem = ms_solve = ms_total = 0.0
         with open(f"{prefix}_log_1.txt", "w") as flog, \
                 open(f"{prefix}_log_nonlinear_1.txt", "w") as fnl:
@@ -114,6 +127,20 @@ def run(inp: MefInput, prefix: str, precision: str = "fp64", ordered: bool = Fal
                             f.write(printnode_line(istep, t, u[n - 1]) + "\n")
                         hist[n].append([istep, t, *u[n - 1]])
                     new_file = False
+                elif op[0] == "pnsf":
+                    if not inp.pnodes:
+                        raise ValueError("No node to print for PNSF!")
+                    tx, txe, txb, flux = s.tform()
+                    poro = s.state()["porosity"][:, None]
+                    mode = "w" if new_sf else "a"
+                    for (stem, name), arr in zip(PNSF_FILES, (tx, txe, txb, flux, poro)):
+                        for n in inp.pnodes:
+                            with open(f"{prefix}_{stem}_node_{n}.txt", mode) as f:
+                                if new_sf:
+                                    f.write(printnode_header(n, name) + "\n")
+                                f.write(printnode_line(istep, t, arr[n - 1]) + "\n")
+                            shist[stem][n].append([istep, t, *arr[n - 1]])
+                    new_sf = False
                 elif op[0] == "solv":
                     istep += 1
                     t = t + dt
@@ -145,6 +172,7 @@ def run(inp: MefInput, prefix: str, precision: str = "fp64", ordered: bool = Fal
                            ("nnode", s.nnode), ("numel", deck.numel)):
                 f.write(f"{key:<10}{v:12d}\n")
         return dict(hist={n: np.array(v) for n, v in hist.items()}, stats=stats,
+                    stress={k: {n: np.array(v) for n, v in d.items()} for k, d in shist.items()},
                     nnode=s.nnode, neq=s.neq, nad=s.nad)
     finally:
         s.close()

@@ -173,6 +173,16 @@ class MefSolver:
                     used=list(info.used[:n]), true_rel=list(info.true_rel[:n]),
                     ms_total=info.ms_total, ms_solve=info.ms_solve, ms_element=info.ms_element)
 
+    def tform(self):
+        """Vertex stresses and Darcy flux of the current state (the pnsf
+        macro's tform_pm + effective_stress): total, Terzaghi and Biot stress
+        as (nnode, 6) arrays (xx yy zz xy yz xz) and the flux as (nnode, 3)."""
+        n = self.nnode
+        tx, txe, txb, flux = np.zeros((n, 6)), np.zeros((n, 6)), np.zeros((n, 6)), np.zeros((n, 3))
+        L.check(self.lib.pm_tform(self.ctx, L.ptr(tx), L.ptr(txe), L.ptr(txb), L.ptr(flux)),
+                "pm_tform")
+        return tx, txe, txb, flux
+
     def set_ordered(self, on: bool = True):
         """Sum every reduction and SpMV row in the reference's sequential
         order (slow; for bit-level parity checks)."""

@@ -25,18 +25,27 @@ def extract(case: str, dest: Path) -> Path:
     return dest / f"{case}.dat"
 
 
+KINDS = ("stress", "stressE", "stressB", "flux")
+
+
 def reference(case: str) -> dict:
     """The reference's results for a case: its logged sizes and, per print
-    node, the up_node history (istep, t, u, v, w, p) of every step."""
+    node, its printed histories: "hist" the up_node rows (istep, t, u, v, w,
+    p), and per kind of KINDS the pnsf rows (istep, t, components)."""
     z = np.load(GOLD / f"{case}_ref.npz")
     out = {k: int(z[k]) for k in ("nnode", "neq", "nad", "numel")}
-    out["hist"] = {int(k[5:]): z[k] for k in z.files if k.startswith("node_")}
+    out["hist"] = {int(k[3:]): z[k] for k in z.files if k.startswith("up_")}
+    for kind in KINDS:
+        out[kind] = {int(k[len(kind) + 1:]): z[k] for k in z.files
+                     if k.startswith(kind + "_") and k[len(kind) + 1:].isdigit()}
     return out
 
 
-def oracle_run(inp, nsteps=None):
+def oracle_run(inp, nsteps=None, stress=None):
     """The oracle's run of the deck's program: {node: (n, 6) history}, the
-    per-step iteration counts and the final nodal u."""
+    per-step iteration counts and the final nodal u.  A dict passed as
+    `stress` receives the pnsf histories ({kind: {node: rows}}, kinds
+    stress, stressE, stressB, flux, poro)."""
     c = inp.cfg
     deck = inp.deck
     st = O.build_structure(deck, reorder=inp.reorder)
@@ -55,6 +64,12 @@ def oracle_run(inp, nsteps=None):
         elif op[0] == "pnup":
             for n in inp.pnodes:
                 hist[n].append([istep, t, *m.u[n - 1]])
+        elif op[0] == "pnsf" and stress is not None:
+            arrs = (*m.tform(), m.porosity[:, None])
+            for kind, a in zip(KINDS + ("poro",), arrs):
+                d = stress.setdefault(kind, {})
+                for n in inp.pnodes:
+                    d.setdefault(n, []).append([istep, t, *a[n - 1]])
         elif op[0] == "solv":
             if nsteps is not None and istep == nsteps:
                 break
@@ -62,6 +77,9 @@ def oracle_run(inp, nsteps=None):
             t = t + m.env.dt
             info = m.step()
             iters.append(info["iters"])
+    if stress is not None:
+        for kind in stress:
+            stress[kind] = {n: np.array(v) for n, v in stress[kind].items()}
     return {n: np.array(v) for n, v in hist.items()}, iters, m.u.copy()
 
 

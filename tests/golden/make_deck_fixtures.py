@@ -3,8 +3,10 @@ reference ships with their outputs (output/poro_mec/elastico/hexa).
 
 For each case it stores
   <case>.tar.gz  the deck and every file it names (insert / setpnode), as is;
-  <case>_ref.npz the reference's own results for it: the up_node histories
-                 (columns istep, t, u, v, w, p) of each print node, and the
+  <case>_ref.npz the reference's own results for it: the printnode
+                 histories of each print node, <kind>_<node> for kind up
+                 (istep, t, u, v, w, p), stress / stressE / stressB (istep,
+                 t, xx yy zz xy yz xz) and flux (istep, t, qx qy qz), and the
                  nnode / neq / nad the reference logged in <case>_t_1.txt.
 These are data (inputs and the reference's outputs); nothing of the
 reference's code is stored.  Run from the repository root:
@@ -49,9 +51,10 @@ def main():
                 tar.addfile(info, io.BytesIO(data))
         arrays = {}
         for fn in sorted(os.listdir(SRC)):
-            m = re.fullmatch(re.escape(case) + r"_up_node_(\d+)\.txt", fn)
+            m = re.fullmatch(re.escape(case) + r"_(up|stress|stressE|stressB|flux)_node_(\d+)\.txt",
+                             fn)
             if m:
-                arrays[f"node_{m.group(1)}"] = read_hist(os.path.join(SRC, fn))
+                arrays[f"{m.group(1)}_{m.group(2)}"] = read_hist(os.path.join(SRC, fn))
         for line in open(os.path.join(SRC, case + "_t_1.txt")):
             w = line.split()
             if len(w) == 2 and w[0] in ("nnode", "neq", "nad", "numel"):

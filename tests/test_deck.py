@@ -5,8 +5,9 @@ The example decks the reference ships with its outputs
 (output/poro_mec/elastico/hexa) pin the reader from two sides: the system
 size the reference logged for each (nnode, neq, nad in its _t_1.txt) must
 come out of the parsed mesh, and the oracle's march of the parsed deck must
-give the reference's own up_node histories, at every one of exemplo1's 600
-steps, to the 10 decimals they were printed with.  A misread coordinate,
+give the reference's own up_node, stress, stressE, stressB and flux
+histories, at every one of exemplo1's 600 steps, to the 10 decimals they
+were printed with.  A misread coordinate,
 connectivity, restraint, face load, material or initial pressure moves
 one of them.  Write -> read round trips and hand-checked small decks cover
 the current grammar (file-based config/solver/gravity/loads), and amdflang
@@ -17,7 +18,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from deckcases import CASES, extract, oracle_run, print_close, reference
+from deckcases import CASES, KINDS, extract, oracle_run, print_close, reference
 from poromec_amd import problem as P
 from poromec_amd.deck import EPS, DeckError, read_deck, truncate, write_deck
 from poromec_amd.mef import fortran_e, printnode_line, read_printnode
@@ -39,7 +40,7 @@ def test_example_deck_settings(tmp_path):
     assert (a.cfg.maxnlit, a.cfg.stop_crit, a.cfg.maxnlit_accept) == (2, 1, True)
     assert a.reorder is False and a.pnodes == [1, 44]
     assert a.deck.gravity.tolist() == [0.0, 0.0, -10.0] and a.deck.gravity_mod == 10.0
-    assert a.program[:3] == [("dt", 3600.0), ("pnup",), ("solv",)]
+    assert a.program[:5] == [("dt", 3600.0), ("pnup",), ("pnsf",), ("solv",), ("pnup",)]
     assert sum(op == ("solv",) for op in a.program) == 600
     # face 1 of element 1 carries load 1: a constant traction (0, 0, -3.9)
     assert a.deck.eload[0].tolist() == [1, 0, 0, 0, 0, 0, 0] and not a.deck.eload[1:].any()
@@ -47,7 +48,7 @@ def test_example_deck_settings(tmp_path):
     assert a.deck.fload[0, :4, 0].tolist() == [0.0, 0.0, 0.0, -3.9]
     # initial pressures, zeroed on the drained (restrained) top
     assert a.u0[1, 3] == 1.0e-3 and a.u0[43, 3] == 1.0e-2 and a.u0[0, 3] == 0.0
-    assert {"pgeo", "pnsf"} <= set(a.skipped)
+    assert "pgeo" in a.skipped and ("pnsf",) in a.program
     b = read_deck(extract("solo1", tmp_path))
     assert b.cfg.tol == EPS  # "pcg 10000 0.0d0": tol 0 -> machine epsilon, as logged
     c = read_deck(extract("solo_cilindro_hexa_1", tmp_path))
@@ -57,16 +58,20 @@ def test_example_deck_settings(tmp_path):
 @pytest.mark.parametrize("case,nsteps", [("exemplo1_mesh0_t2", 600), ("solo1", 4),
                                          ("solo_cilindro_hexa_1", 4)])
 def test_oracle_reproduces_reference_histories(tmp_path, case, nsteps):
+    """Displacements and pressure (pnup) and the vertex stresses and flux
+    (pnsf: total, Terzaghi, Biot, Darcy) the reference printed."""
     inp = read_deck(extract(case, tmp_path))
-    hist, iters, _ = oracle_run(inp, nsteps)
+    stress = {}
+    hist, iters, _ = oracle_run(inp, nsteps, stress=stress)
     ref = reference(case)
-    for n, h in hist.items():
-        r = ref["hist"][n][: nsteps + 1]
-        assert h.shape == r.shape
-        assert np.array_equal(h[:, 0], r[:, 0])
-        assert np.allclose(h[:, 1], r[:, 1], rtol=1e-12)
-        ok = print_close(h[:, 2:], r[:, 2:])
-        assert ok.all(), (n, np.argwhere(~ok)[:5])
+    for kind, mine in [("hist", hist)] + [(k, stress[k]) for k in KINDS]:
+        for n, h in mine.items():
+            r = ref[kind][n][: nsteps + 1]
+            assert h.shape == r.shape, (kind, n)
+            assert np.array_equal(h[:, 0], r[:, 0])
+            assert np.allclose(h[:, 1], r[:, 1], rtol=1e-12)
+            ok = print_close(h[:, 2:], r[:, 2:])
+            assert ok.all(), (kind, n, np.argwhere(~ok)[:5])
 
 
 def test_write_read_round_trip(tmp_path):

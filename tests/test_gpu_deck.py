@@ -5,9 +5,9 @@ summation order) and the reference's own printed results."""
 import numpy as np
 import pytest
 
-from deckcases import extract, oracle_run, print_close, reference
+from deckcases import KINDS, extract, oracle_run, print_close, reference
 from poromec_amd.deck import read_deck, truncate
-from poromec_amd.mef import read_printnode, run
+from poromec_amd.mef import PNSF_FILES, read_printnode, run
 
 pytestmark = pytest.mark.gpu
 
@@ -24,6 +24,14 @@ def _check_files(prefix, inp, nsteps, res):
         # the text is the run's history to the 10 significant digits printed
         assert np.allclose(u, res["hist"][n][:, 2:], rtol=5.01e-10, atol=1e-300)
         hist[n] = np.column_stack([st, t, u])
+    for stem, name in PNSF_FILES:
+        for n in inp.pnodes:
+            path = f"{prefix}_{stem}_node_{n}.txt"
+            lines = open(path).read().splitlines()
+            assert lines[0] == f"# Variacao {name} no tempo no ponto{n:9d}"
+            assert len(lines) == nsteps + 2
+            st, t, v = read_printnode(path)
+            assert np.allclose(v, res["stress"][stem][n][:, 2:], rtol=5.01e-10, atol=1e-300)
     for suffix in ("_log_1.txt", "_log_nonlinear_1.txt", "_t_1.txt"):
         assert open(prefix + suffix).readline()
     return hist
@@ -37,8 +45,20 @@ def test_deck_run_matches_oracle_and_reference(tmp_path, case, nsteps):
     prefix = str(tmp_path / case)
     res = run(inp, prefix)
     hist = _check_files(prefix, inp, nsteps, res)
-    ohist, oiters, _ = oracle_run(inp, nsteps)
+    ostress = {}
+    ohist, oiters, _ = oracle_run(inp, nsteps, stress=ostress)
     ref = reference(case)
+    # stresses and flux: 1e-9 of the largest value of each kind the print
+    # nodes see; the reference's printed values to their 10 decimals
+    for kind in KINDS + ("poro",):
+        allo = np.concatenate([ostress[kind][n][:, 2:] for n in inp.pnodes])
+        tol = 1e-9 * np.abs(allo).max() + 1e-300
+        for n in inp.pnodes:
+            g = res["stress"][kind][n]
+            assert (np.abs(g[:, 2:] - ostress[kind][n][:, 2:]) <= tol).all(), (kind, n)
+            if kind in ref:
+                ok = print_close(g[:, 2:], ref[kind][n][: nsteps + 1, 2:])
+                assert ok.all(), (kind, n, np.argwhere(~ok)[:5])
     # 1e-9 of the largest displacement / pressure the print nodes see (a
     # component that is zero by symmetry carries rounding noise only)
     allo = np.concatenate([ohist[n][:, 2:] for n in inp.pnodes])
@@ -66,9 +86,12 @@ def test_deck_ordered_bitwise(tmp_path):
     nsteps = 40
     inp = truncate(read_deck(extract("exemplo1_mesh0_t2", tmp_path)), nsteps)
     res = run(inp, str(tmp_path / "ord"), ordered=True)
-    ohist, oiters, _ = oracle_run(inp, nsteps)
+    ostress = {}
+    ohist, oiters, _ = oracle_run(inp, nsteps, stress=ostress)
     for n in inp.pnodes:
         assert np.array_equal(res["hist"][n], ohist[n]), n
+        for kind in KINDS + ("poro",):
+            assert np.array_equal(res["stress"][kind][n], ostress[kind][n]), (kind, n)
     assert [s["iters"] for s in res["stats"]] == oiters
 
 

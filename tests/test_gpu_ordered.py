@@ -189,3 +189,27 @@ def test_ordered_march_bitwise(which):
         assert np.array_equal(g["dp"], m.dp), k
         assert np.array_equal(g["porosity"], m.porosity), k
     s.close()
+
+
+@pytest.mark.parametrize("which", ["column", "gravity_box"])
+def test_tform_bitwise(which):
+    """Vertex stresses and Darcy flux (the pnsf quantities) of a random
+    state: the element-order gather sums as tform_pm does, so the kernel's
+    results are the oracle's bit for bit, in the default mode too."""
+    from poromec_amd.solver import MefSolver
+    d = _deck_of(which)
+    st = O.build_structure(d)
+    s = MefSolver(d)
+    m = O.OracleModel(d, st)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((st["nnode"], 4)) * 1e-3
+    dp = rng.standard_normal(st["nnode"]) * 1e-4
+    u[d.nnodev:, 3] = 0.0
+    dp[d.nnodev:] = 0.0
+    m.u[:] = u
+    m.dp[:] = dp
+    s.set_state(u=u, u0=np.zeros_like(u), dp=dp, t=0.0)
+    for name, g, o in zip(("tx", "txe", "txb", "flux"), s.tform(), m.tform()):
+        assert np.array_equal(g, o), (name, np.abs(g - o).max())
+        assert np.abs(o[: d.nnodev]).max() > 0
+    s.close()

@@ -1,12 +1,14 @@
 """Pins the oracle's hexa20/8 element (elmt17_pm, Elmlib03.f:1822-2636).
 
 * bit-for-bit against the reference's own Fortran elmt17_pm (built from the
-  reference sources into oracle/_ref) for isw 2, 4 and 7, with gravity,
+  reference sources into oracle/_ref) for isw 2, 3, 4 and 7, with gravity,
   type-40 tractions on two faces and a type-43 flux;
 * closed forms that do not go through either code: rigid-body modes of Kuu,
   partition of unity in Kpu, the exact trilinear mass and Laplacian of a
   unit cube in Kpp, the serendipity face-load split (-1/12 corners, 1/3
-  midsides), isw 2 / isw 4 consistency, and isw 7 on a linear field.
+  midsides), isw 2 / isw 4 consistency, isw 7 on a linear field and isw 3
+  (vertex stresses and Darcy flux) on linear displacement and pressure
+  fields.
 """
 import numpy as np
 import pytest
@@ -36,7 +38,7 @@ def env(dt=0.07, t=0.3, gravity=(0.0, 0.0, 0.0), gmod=10.0, load=None, fload=Non
     return O.make_env(load, fload, dt, t, gravity, gmod)
 
 
-@pytest.mark.parametrize("isw", [2, 4, 7])
+@pytest.mark.parametrize("isw", [2, 3, 4, 7])
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_element_bitwise_vs_reference(ref, isw, seed):
     rng = np.random.default_rng(seed)
@@ -53,9 +55,12 @@ def test_element_bitwise_vs_reference(ref, isw, seed):
     iqm = np.array([1, 0, 0, 0, 1, 0, 0], np.int32)
     iqh = np.array([0, 0, 1, 0, 0, 0, 0], np.int32)
     grav = (0.1, -0.3, -9.81)
-    rc, p, s = O.elmt17(e, iqm, iqh, xl, ul, dpl, isw, env(0.07, 0.3, grav, 9.81, load, fload))
+    p0 = np.zeros(120) if isw == 3 else None  # isw 3 fills 8 x (6 + 6 + 3) values
+    rc, p, s = O.elmt17(e, iqm, iqh, xl, ul, dpl, isw, env(0.07, 0.3, grav, 9.81, load, fload),
+                        p=p0)
     assert rc == 0
-    pr, sr = O.ref_elmt17(e, iqm, iqh, xl, ul, dpl, isw, 0.07, 0.3, grav, 9.81, load, fload)
+    pr, sr = O.ref_elmt17(e, iqm, iqh, xl, ul, dpl, isw, 0.07, 0.3, grav, 9.81, load, fload,
+                          p=p0)
     assert np.array_equal(p, pr)
     if isw == 2:
         assert np.array_equal(s, sr)
@@ -170,3 +175,38 @@ def test_isw7_linear_field():
     z7 = np.zeros(7, np.int32)
     rc, p, _ = O.elmt17(e, z7, z7, xl, u, np.zeros(8), 7, env())
     assert np.allclose(p[:8], 0.9 * (a + b + c) + u[60:] / 500.0, rtol=1e-12, atol=1e-14)
+
+
+def test_isw3_linear_fields():
+    """A linear displacement field has the constant strain sym(G) at every
+    vertex, so the vertex stress is D sym(G) - alpha dp (total) and that plus
+    alpha p (Biot); a linear pressure g.x + p0 has the Darcy flux
+    perm (rho_f grav - g) everywhere."""
+    rng = np.random.default_rng(11)
+    xl = distorted(rng)
+    E, nu, alpha, kw = 20.0, 0.3, 0.9, 0.1
+    e = P.material_vector([E], nu, kw, M=500.0, alpha=alpha)[0]
+    G = rng.standard_normal((3, 3)) * 1e-3
+    X = node_coords(xl)
+    gp, p0 = np.array([0.3, -0.2, 0.45]), 0.7
+    u = np.zeros(68)
+    u[:60] = (X @ G.T).ravel()
+    u[60:] = xl @ gp + p0
+    dp = rng.standard_normal(8)
+    grav = (0.1, -0.3, -9.81)
+    z7 = np.zeros(7, np.int32)
+    rc, p, _ = O.elmt17(e, z7, z7, xl, u, dp, 3, env(gravity=grav, gmod=9.81), p=np.zeros(120))
+    assert rc == 0
+    eps = np.array([G[0, 0], G[1, 1], G[2, 2], G[0, 1] + G[1, 0], G[1, 2] + G[2, 1],
+                    G[0, 2] + G[2, 0]])
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    sig = np.concatenate([lam * eps[:3].sum() + 2 * mu * eps[:3], mu * eps[3:]])
+    perm = e[2] / (e[6] * 1e-6 * 9.81)
+    q = perm * (e[6] * 1e-6 * np.array(grav) - gp)
+    for i in range(8):
+        tot = sig - alpha * dp[i] * np.array([1, 1, 1, 0, 0, 0])
+        biot = tot + alpha * u[60 + i] * np.array([1, 1, 1, 0, 0, 0])
+        assert np.allclose(p[6 * i:6 * i + 6], tot, rtol=1e-11, atol=1e-13)
+        assert np.allclose(p[48 + 6 * i:54 + 6 * i], biot, rtol=1e-11, atol=1e-13)
+        assert np.allclose(p[96 + 3 * i:99 + 3 * i], q, rtol=1e-11, atol=1e-20)
