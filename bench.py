@@ -245,6 +245,69 @@ def adjoint_form(args):
                       "fd_rel_err_median": float(np.median(errs)) if errs else None}), flush=True)
 
 
+def tform_form(args):
+    """The pnsf stress recovery (pm_tform) on the n^3 box: vertex total,
+    Terzaghi and Biot stresses and Darcy flux of a smooth state, timed per
+    call (the kernel, no host copies), against its compulsory HBM bytes; the
+    oracle's tform_pm restatement timed on a small box beside it."""
+    import ctypes as C
+
+    from oracle import oracle as O
+    from poromec_amd import problem as P
+    from poromec_amd.solver import MefSolver
+
+    def state(deck, nnode):
+        ix = np.arange(nnode, dtype=np.float64)
+        u = np.stack([1e-3 * np.sin(ix * 1e-3), 1e-3 * np.cos(ix * 7e-4), -1e-3 * np.sin(ix * 3e-4),
+                      np.cos(ix * 1e-4)], axis=1)
+        u[deck.nnodev:, 3] = 0.0
+        dp = 1e-2 * np.sin(ix * 5e-4)
+        dp[deck.nnodev:] = 0.0
+        return u, dp
+
+    n = args.n
+    deck = P.heterogeneous_box_deck(n, seed=args.seed, dt=0.1, nsteps=1)
+    s = MefSolver(deck)
+    u, dp = state(deck, s.nnode)
+    s.set_state(u=u, u0=u, dp=dp, t=0.0)
+    tx = s.tform()[0]  # first call builds the node -> (element, corner) lists
+    null = C.c_void_p()
+    reps = max(args.spmv_reps, 5)
+    s.lib.pm_tform(s.ctx, null, null, null, null)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rc = s.lib.pm_tform(s.ctx, null, null, null, null)
+        assert rc == 0, rc
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    npairs = 8 * deck.numel
+    nbytes = (s.nnode * (4 + 1 + 21) * 8 + deck.nnodev * 3 * 8 + deck.numel * (20 + 1) * 4
+              + (s.nnode + 1) * 8 + npairs * 4)
+    log(f"tform {n}^3: {ms:.3f} ms per call, {nbytes / ms / 1e6:.0f} GB/s compulsory")
+    # oracle restatement on a small box, per element
+    m_small = max(4, min(n, 12))
+    dsm = P.heterogeneous_box_deck(m_small, seed=args.seed, dt=0.1, nsteps=1)
+    m = O.OracleModel(dsm)
+    us, dps = state(dsm, m.st["nnode"])
+    m.u[:] = us
+    m.dp[:] = dps
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < 2.0 or k < 2:
+        m.tform()
+        k += 1
+    cpu_s = (time.perf_counter() - t0) / k
+    cpu_per_elem = cpu_s / dsm.numel
+    print(json.dumps({"form": "tform", "n": n, "numel": deck.numel, "nnode": s.nnode,
+                      "ms_per_call": ms, "compulsory_bytes": nbytes,
+                      "GBps_compulsory": nbytes / ms / 1e6, "frac_of_8TBps": nbytes / ms / 8e9,
+                      "elements_per_s": deck.numel / ms * 1e3,
+                      "cpu_oracle": {"box": m_small, "s_per_call": cpu_s,
+                                     "elements_per_s": 1.0 / cpu_per_elem, "cores": 1,
+                                     "kind": "port"},
+                      "max_abs_tx": float(np.abs(tx).max())}), flush=True)
+    s.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,7 +318,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--cpu-sample-n", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", choices=["precision", "batch", "adjoint"], default="precision",
+    ap.add_argument("--form", choices=["precision", "batch", "adjoint", "tform"],
+                    default="precision",
                     help="precision (default, the metric's workload), batch or adjoint form")
     ap.add_argument("--nbox", type=int, default=256)
     ap.add_argument("--nmin", type=int, default=12)
@@ -280,6 +344,8 @@ def main():
         return batch_form(args)
     if args.form == "adjoint":
         return adjoint_form(args)
+    if args.form == "tform":
+        return tform_form(args)
 
     from poromec_amd import problem as P
     from poromec_amd.solver import MefSolver, SolverConfig

@@ -15,7 +15,7 @@
 #include "pm_elemlib.h"
 
 struct TformArgs {
-  int nnode;
+  int n0, n1;          // nodes of this launch
   const int64_t *ptr;  // (nnode + 1) (element, corner) pairs of each node
   const int *src;      // element * 8 + corner, ascending element
   const int *ix;       // (numel, 20) 0-based
@@ -30,8 +30,10 @@ struct TformArgs {
 };
 
 __global__ __launch_bounds__(256) void k_tform(TformArgs A) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= A.nnode) return;
+  // each XCD takes a contiguous range of nodes, so the elements that
+  // neighbouring nodes share are gathered through one L2
+  const int n = A.n0 + pm_xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (n >= A.n1) return;
   double t[6] = {0, 0, 0, 0, 0, 0}, tb[6] = {0, 0, 0, 0, 0, 0}, f[3] = {0, 0, 0};
   const int64_t p0 = A.ptr[n], p1 = A.ptr[n + 1];
   for (int64_t q = p0; q < p1; q++) {
@@ -168,7 +170,6 @@ int pm_tform_dev(pm_ctx *c, double *tx, double *txe, double *txb, double *flux) 
   PM_TRY(ensure_incidence(c));
   const int nn = c->mesh.nnode;
   TformArgs A;
-  A.nnode = nn;
   A.ptr = c->d_tf_ptr;
   A.src = c->d_tf_src;
   A.ix = c->d_ix;
@@ -185,7 +186,20 @@ int pm_tform_dev(pm_ctx *c, double *tx, double *txe, double *txb, double *flux) 
   A.flux = A.txb + (size_t)nn * 6;
   A.err = c->d_elem_err;
   PM_CHECK_HIP(hipMemsetAsync(c->d_elem_err, 0, sizeof(int), c->stream));
-  hipLaunchKernelGGL(k_tform, dim3((nn + 255) / 256), dim3(256), 0, c->stream, A);
+  // the vertices (nodes 1..nnodev; promotion numbers the mid-edge nodes
+  // after them) carry all the element work: their launch is spread over the
+  // 8 XCDs on its own (a grid of a multiple of 8 workgroups); the mid-edge
+  // nodes only receive zeros and the pressure shift of txe
+  const int nv = c->mesh.nnodev;
+  A.n0 = 0;
+  A.n1 = nv;
+  const int gv = (((nv + 255) / 256) + 7) & ~7;
+  hipLaunchKernelGGL(k_tform, dim3(gv), dim3(256), 0, c->stream, A);
+  if (nn > nv) {
+    A.n0 = nv;
+    A.n1 = nn;
+    hipLaunchKernelGGL(k_tform, dim3((nn - nv + 255) / 256), dim3(256), 0, c->stream, A);
+  }
   PM_CHECK_HIP(hipGetLastError());
   int err = 0;
   PM_CHECK_HIP(hipMemcpyAsync(&err, c->d_elem_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
