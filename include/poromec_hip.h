@@ -126,7 +126,9 @@ int pm_get_numbering(pm_ctx *ctx, int *ix20, int *id, int *perm);
 
 /* Materials, face loads, load table and gravity (materials / elmtloads /
  * loads / gravity macros, Rdata.f:1132-1180, 1487, 3543).  Element type 17
- * (elmt17_pm) only.  load: load(3,200), fload: fload(1000,5,200) of load.fi. */
+ * (elmt17_pm) only.  load: load(3,200), fload: fload(1000,5,200) of load.fi.
+ * Like the mesh macro (initial_porosity, Main.f:820-828) it also sets the
+ * nodal porosity to the vertex average of e(8). */
 int pm_set_materials(pm_ctx *ctx, int numat, const double *e, const int *eload,
                      const int *eloadp, const int *load, const double *fload,
                      const double *gravity, double gravity_mod);

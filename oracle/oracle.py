@@ -103,6 +103,7 @@ def _load_lib():
     L.orc_step.restype = C.c_int
     L.orc_step.argtypes = [C.POINTER(_Model), C.POINTER(_Cfg), C.POINTER(OracleEnv),
                            C.POINTER(_StepInfo)]
+    L.orc_initial_porosity.argtypes = [C.POINTER(_Model)]
     L.orc_tform.restype = C.c_int
     L.orc_tform.argtypes = [C.POINTER(_Model), _f64p, _f64p, _f64p, _f64p, C.POINTER(OracleEnv)]
     L.orc_tload.restype = C.c_double
@@ -215,6 +216,7 @@ class OracleModel:
                           ("u0", self.u0), ("dp", self.dp), ("porosity", self.porosity),
                           ("b0", self.b0), ("b", self.b), ("x0", self.x0), ("m", self.m)):
             setattr(md, name, arr.ctypes.data)
+        lib.orc_initial_porosity(C.byref(md))  # as the mesh macro does
         self.t = 0.0
 
     def assemble(self, isw=2, lhs=True, rhs=True, bvec=None):

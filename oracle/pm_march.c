@@ -8,6 +8,7 @@
  *                      gather :307-381) + csr      Assbly.f:218-304
  *   orc_porosity       porosity_form               Pform.f:969-1176
  *   orc_tform          tform_pm, effective_stress  Pform.f:433-700, 2487-2503
+ *   orc_initial_porosity initial_porosity          Pform.f:1331-1376
  *   orc_pload          pload_pm                    Pload.f:1-92
  *   orc_delta_predict  delta_predict_pm            Predict.f:117-158
  *   orc_delta_update   delta_update_pm             Predict.f:159-215
@@ -92,6 +93,25 @@ int orc_pform(orc_model *md, int isw, int lhs, int rhs, double *bvec, const orc_
   }
   free(sl);
   return rc;
+}
+
+void orc_initial_porosity(orc_model *md) {
+  int *ic = calloc((size_t)md->nnode, sizeof(int));
+  for (int i = 0; i < md->nnode; i++) md->porosity[i] = 0.0;
+  for (int nel = 0; nel < md->numel; nel++) {
+    int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
+    double poro = md->e[(size_t)(ma - 1) * ORC_PROP + 7];
+    for (int i = 0; i < ORC_NENV; i++) {
+      int no = md->ix[(size_t)nel * NEN1 + i];
+      if (no <= 0) continue;
+      ic[no - 1]++;
+      md->porosity[no - 1] = md->porosity[no - 1] + poro;
+    }
+  }
+  /* a vertex of no element keeps 0 (the reference would divide 0 by 0) */
+  for (int i = 0; i < md->nnode; i++)
+    if (md->fnno[i] == 1 && ic[i] > 0) md->porosity[i] = md->porosity[i] / ic[i];
+  free(ic);
 }
 
 int orc_tform(orc_model *md, double *tx, double *txe, double *txb, double *flux,

@@ -99,6 +99,8 @@ void orc_csrc_fill(const int *id, const int *num, const int64_t *ip, const int *
 /* ---- element loop + assembly (Pform.f, Assbly.f) ---- */
 int orc_pform(orc_model *md, int isw, int lhs, int rhs, double *bvec, const orc_env *env);
 void orc_porosity(orc_model *md, double *dporo, int *ic, const orc_env *env);
+/* initial_porosity (Pform.f:1331-1376): vertex average of e(8) */
+void orc_initial_porosity(orc_model *md);
 /* tform_pm (Pform.f:433-700) + effective_stress (Pform.f:2487-2503): vertex
  * averages of total stress tx (6 x nnode), Terzaghi tx + p (txe), Biot's
  * tx + alpha p (txb) and the Darcy flux (3 x nnode); other nodes get zero. */

@@ -471,6 +471,25 @@ extern "C" int pm_set_materials(pm_ctx *c, int numat, const double *e, const int
   c->has_face_loads = any != 0;
   for (int d = 0; d < 3; d++) c->gravity[d] = gravity ? gravity[d] : 0.0;
   c->gravity_mod = gravity_mod;
+  // initial_porosity (Pform.f:1331-1376, run by the mesh macro, Main.f:820-828):
+  // every vertex starts at the average of e(8) over its elements, summed in
+  // element order
+  std::vector<double> poro((size_t)m.nnode, 0.0);
+  std::vector<int> ic((size_t)m.nnode, 0);
+  for (int el = 0; el < m.numel; el++) {
+    const int ma = m.ix20[(size_t)el * 21 + 20];
+    const double p = e[(size_t)(ma - 1) * PM_PROP + 7];
+    for (int i = 0; i < 8; i++) {
+      const int no = m.ix20[(size_t)el * 21 + i];
+      if (no <= 0) continue;
+      ic[no - 1]++;
+      poro[no - 1] = poro[no - 1] + p;
+    }
+  }
+  for (int n = 0; n < m.nnode; n++)
+    if (m.fnno[n] == 1 && ic[n] > 0) poro[n] = poro[n] / ic[n];
+  PM_CHECK_HIP(hipMemcpyAsync(c->d_poro, poro.data(), sizeof(double) * poro.size(),
+                              hipMemcpyHostToDevice, c->stream));
   PM_CHECK_HIP(hipStreamSynchronize(c->stream));
   c->have_materials = true;
   return PM_OK;

@@ -152,3 +152,20 @@ def test_terzaghi_column():
             worst = max(worst, np.abs(p - pa).max() / p0)
     # backward Euler (dt = 0.25 s) on 10 quadratic elements
     assert worst < 0.015, worst
+
+
+def test_initial_porosity_vertex_average():
+    """initial_porosity: a vertex starts at the mean e(8) of its elements.
+    Two elements side by side with porosities 0.2 and 0.4: the 4 shared
+    vertices get 0.3, the others their element's value, mid-edge nodes 0."""
+    d = P.column_deck(2, 1, 1)
+    d.e = np.vstack([d.e[:1], d.e[:1]])
+    d.e[0, 7], d.e[1, 7] = 0.2, 0.4
+    d.ix[:, 20] = [1, 2]
+    m = O.OracleModel(d)
+    shared = set(d.ix[0, :8]) & set(d.ix[1, :8])
+    assert len(shared) == 4
+    for v in range(1, d.nnodev + 1):
+        want = 0.3 if v in shared else (0.2 if v in d.ix[0, :8] else 0.4)
+        assert abs(m.porosity[v - 1] - want) < 1e-15, v
+    assert not m.porosity[d.nnodev:].any()
