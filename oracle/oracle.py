@@ -42,6 +42,7 @@ class _Model(C.Structure):
         ("ia", C.c_void_p), ("ja", C.c_void_p), ("ad", C.c_void_p), ("al", C.c_void_p),
         ("u", C.c_void_p), ("u0", C.c_void_p), ("dp", C.c_void_p), ("porosity", C.c_void_p),
         ("b0", C.c_void_p), ("b", C.c_void_p), ("x0", C.c_void_p), ("m", C.c_void_p),
+        ("ie", C.c_void_p), ("vprop", C.c_int * 4), ("vpropel", C.c_void_p),
     ]
 
 
@@ -104,6 +105,20 @@ def _load_lib():
     L.orc_step.argtypes = [C.POINTER(_Model), C.POINTER(_Cfg), C.POINTER(OracleEnv),
                            C.POINTER(_StepInfo)]
     L.orc_initial_porosity.argtypes = [C.POINTER(_Model)]
+    L.orc_initial_prop.restype = C.c_int
+    L.orc_initial_prop.argtypes = [C.POINTER(_Model), _f64p, C.c_int]
+    L.orc_update_prop.restype = C.c_int
+    L.orc_update_prop.argtypes = [C.POINTER(_Model), C.POINTER(OracleEnv)]
+    L.orc_elmt18.restype = C.c_int
+    L.orc_elmt18.argtypes = [_f64p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                             C.c_int, C.POINTER(OracleEnv)]
+    L.orc_konzey_carman.restype = C.c_double
+    L.orc_konzey_carman.argtypes = [C.c_double, C.c_double, C.c_double]
+    dp_ = C.POINTER(C.c_double)
+    L.orc_hashin_shtrikman.argtypes = [C.c_double, C.c_double, C.c_double, dp_, dp_, dp_, dp_]
+    L.orc_ks_and_mus.restype = C.c_int
+    L.orc_ks_and_mus.argtypes = [C.c_double, C.c_double, C.c_double, dp_, dp_]
+    L.orc_extrapol_hexa20.argtypes = [_f64p, _f64p, C.c_int]
     L.orc_tform.restype = C.c_int
     L.orc_tform.argtypes = [C.POINTER(_Model), _f64p, _f64p, _f64p, _f64p, C.POINTER(OracleEnv)]
     L.orc_tload.restype = C.c_double
@@ -143,6 +158,21 @@ def elmt17(e, iqm, iqh, xl, ul, dpl, isw, env, p=None):
     return rc, p, s.reshape(NST, NST).T  # s stored column-major
 
 
+def elmt18(e, iqm, iqh, xl, ul, dpl, vp, isw, env, p=None):
+    """One elmt18_pm call; returns (rc, p, s[68,68] as s[i, j], vp after)."""
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    iqm = np.ascontiguousarray(iqm, dtype=np.int32)
+    iqh = np.ascontiguousarray(iqh, dtype=np.int32)
+    xl = np.ascontiguousarray(xl, dtype=np.float64).reshape(-1)
+    ul = np.ascontiguousarray(ul, dtype=np.float64)
+    dpl = np.ascontiguousarray(dpl, dtype=np.float64)
+    vp = np.ascontiguousarray(vp, dtype=np.float64).reshape(-1).copy()
+    p = np.zeros(max(NST, 120)) if p is None else np.ascontiguousarray(p, np.float64).copy()
+    s = np.zeros(NST * NST)
+    rc = lib.orc_elmt18(e, iqm, iqh, xl, ul, dpl, p, s, vp, isw, C.byref(env))
+    return rc, p, s.reshape(NST, NST).T, vp.reshape(64, 7)
+
+
 def build_structure(deck, reorder=True):
     """Runs promotion, mk_bound_quad, graph, RCM, numbering and the CSRC
     pattern exactly as the mesh macro does.  Returns a dict of arrays."""
@@ -180,12 +210,13 @@ class OracleModel:
     """All state of the oracle's time march for one deck."""
 
     def __init__(self, deck, st=None, solver=11, precond=2, solvtol=1e-10, maxit=50000,
-                 ngram=150, maxnlit=500, tol=1e-4, stop_crit=1, maxnlit_accept=False):
+                 ngram=150, maxnlit=500, tol=1e-4, stop_crit=1, maxnlit_accept=False,
+                 ie=None, vprop=None):
         self.deck = deck
         self.st = st if st is not None else build_structure(deck)
         st = self.st
         nnode, neq, nad = st["nnode"], st["neq"], st["nad"]
-        self.e = np.ascontiguousarray(deck.e, dtype=np.float64)
+        self.e = np.array(deck.e, dtype=np.float64)  # initial_prop rewrites it
         self.eload = np.ascontiguousarray(deck.eload, dtype=np.int32)
         self.eloadp = np.ascontiguousarray(deck.eloadp, dtype=np.int32)
         self.x = np.zeros((nnode, 3))
@@ -217,6 +248,20 @@ class OracleModel:
                           ("b0", self.b0), ("b", self.b), ("x0", self.x0), ("m", self.m)):
             setattr(md, name, arr.ctypes.data)
         lib.orc_initial_porosity(C.byref(md))  # as the mesh macro does
+        # elmt18_pm materials (ie = 18) and the conseq flags: initial_prop
+        # rewrites e and sets every integration point's properties
+        self.vpropel = np.zeros((st["numel"], 64, 7))
+        md.vpropel = self.vpropel.ctypes.data
+        if ie is not None:
+            self.ie = np.ascontiguousarray(ie, dtype=np.int32)
+            md.ie = self.ie.ctypes.data
+        if vprop is not None:
+            vp = [int(bool(v)) for v in vprop]
+            if any(vp[1:]):
+                vp[0] = 1
+            md.vprop[:] = vp
+            if vp[0] and lib.orc_initial_prop(C.byref(md), self.e, self.e.shape[0]):
+                raise RuntimeError("initial_prop: ks_and_mus has no solution")
         self.t = 0.0
 
     def assemble(self, isw=2, lhs=True, rhs=True, bvec=None):
@@ -286,6 +331,43 @@ def ref_set_common(R, dt, t, gravity, gravity_mod, load, fload):
     buf[: nload * 4] = np.ascontiguousarray(load, dtype=np.int32).view(np.uint8).ravel()
     buf[nload * 4: nload * 4 + 1000 * 5 * 200 * 8] = np.ascontiguousarray(
         fload, dtype=np.float64).view(np.uint8).ravel()
+
+
+def ref_elmt18(e, iqm, iqh, xl, ul, dpl, vp, isw, dt, t=0.0, gravity=(0.0, 0.0, 0.0),
+               gravity_mod=10.0, load=None, fload=None):
+    """Calls the reference's elmt18_pm directly.  Returns (p, s[i, j], vp)."""
+    R = load_ref()
+    if R is None:
+        raise RuntimeError("reference-subset library not built")
+    if load is None:
+        load = np.zeros((200, 3), dtype=np.int32)
+        fload = np.zeros((200, 5, 1000))
+    ref_set_common(R, dt, t, gravity, gravity_mod, load, fload)
+    e = np.ascontiguousarray(e, dtype=np.float64).copy()
+    iqm = np.ascontiguousarray(iqm, dtype=np.int32).copy()
+    iqh = np.ascontiguousarray(iqh, dtype=np.int32).copy()
+    xl = np.ascontiguousarray(xl, dtype=np.float64).reshape(-1).copy()
+    ul = np.ascontiguousarray(ul, dtype=np.float64).copy()
+    dpl = np.ascontiguousarray(dpl, dtype=np.float64).copy()
+    vp = np.ascontiguousarray(vp, dtype=np.float64).reshape(-1).copy()
+    p = np.zeros(max(NST, 120))
+    s = np.zeros(NST * NST)
+    txn = np.zeros(6 * NEN)
+    ndm, nst, nel, isw_, block_pu = C.c_int(3), C.c_int(NST), C.c_int(1), C.c_int(isw), C.c_int(0)
+    dptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    R.elmt18_pm_(dptr(e), dptr(iqm), dptr(iqh), dptr(xl), dptr(ul), dptr(dpl), dptr(p), dptr(s),
+                 dptr(txn), dptr(vp), C.byref(ndm), C.byref(nst), C.byref(nel), C.byref(isw_),
+                 C.byref(block_pu))
+    return p, s.reshape(NST, NST).T, vp.reshape(64, 7)
+
+
+def ref_scalar(name, *args):
+    """Calls a reference real*8 function of real*8 arguments."""
+    R = load_ref()
+    f = getattr(R, name + "_")
+    f.restype = C.c_double
+    vals = [C.c_double(a) for a in args]
+    return f(*[C.byref(v) for v in vals])
 
 
 def ref_elmt17(e, iqm, iqh, xl, ul, dpl, isw, dt, t=0.0, gravity=(0.0, 0.0, 0.0),

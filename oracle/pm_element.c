@@ -338,9 +338,32 @@ double orc_tload(int nc, double t, const double *x, double u, double *vc, const 
  * Returns 0, -1 on a non-positive Jacobian (jacob3d_m stops there), -2 on an
  * unsupported face-load type.  p must hold >= 68 values (120 for isw 3), s
  * 68*68. */
-int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *x,
-               const double *u, const double *dp, double *p, double *s, int isw,
-               const orc_env *env) {
+/* prop_porosity (Poromec.f:165-200): D-matrix constants, Biot moduli and
+ * permeability of one integration point from its variable properties
+ * vp(1..7) = porosity, permeability, density, K, mu, 1/M, alpha.  div43 and
+ * div23 are the reference's 15-digit constants. */
+static void prop_porosity(double *a, double *b, double *c, double *ibiot, double *cbiot,
+                          double *perm, const double *vp, double fluid_sw) {
+  const double div43 = 0.133333333333333e+01, div23 = 0.666666666666667;
+  double modK = vp[3], mu = vp[4];
+  *ibiot = vp[5];
+  *cbiot = vp[6];
+  *a = modK + div43 * mu;
+  *b = (modK - div23 * mu) / *a;
+  *c = mu / *a;
+  *perm = vp[1] / fluid_sw;
+}
+
+/* The hexa20/8 Biot element, isw 2, 3, 4, 7: elmt17_pm when vp is NULL;
+ * with vp the per-integration-point properties (7 x 64) of elmt18_pm, whose
+ * isw 2 and 4 differ from elmt17_pm only there (Elmlib03.f:2872-3067 against
+ * 2022-2205, 3167-3293 against 2263-2372): prop_porosity at each point, Kpp
+ * weighted by w dt perm, the isw 4 flux by w dt rho_f perm and w dt perm,
+ * and in isw 4 the point index advancing once per ly (Elmlib03.f:3171-3173),
+ * so the 64 points read the properties of points 1..16, 4 each. */
+static int elmt_hexa20(const double *e, const int *iqm, const int *iqh, const double *x,
+                       const double *u, const double *dp, double *p, double *s, int isw,
+                       const orc_env *env, const double *vp) {
   double hu[20], hux[20], huy[20], huz[20], hp[8], hpx[8], hpy[8], hpz[8];
   double xj[3][3], xji[3][3], det, w, wt1, wt2, wt3;
   double epsi[6], txi[6];
@@ -362,12 +385,17 @@ int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *x,
       for (int j = 1; j <= nst; j++) S(j, i) = 0.0;
       P(i) = 0.0;
     }
+    int inpi = 0;
     for (int lz = 0; lz < 4; lz++) {
       double ti = PG4[lz];
       for (int ly = 0; ly < 4; ly++) {
         double si = PG4[ly];
         for (int lx = 0; lx < 4; lx++) {
+          inpi++;
           double ri = PG4[lx];
+          if (vp)
+            prop_porosity(&a, &b, &c, &imod_biot, &coef_biot, &perm, vp + 7 * (inpi - 1),
+                          fluid_sw);
           sfhexa8(hp, hpx, hpy, hpz, ri, si, ti, 1, 1);
           if (jacob3d(hpx, hpy, hpz, xj, xji, x, &det, 8, 1)) return -1;
           w = WG4[lx] * WG4[ly] * WG4[lz] * det;
@@ -403,7 +431,7 @@ int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *x,
             }
           }
           wt1 = w * imod_biot;
-          wt2 = w * dt_perm;
+          wt2 = vp ? w * env->dt * perm : w * dt_perm;
           for (int i = 1; i <= 8; i++) {
             int l = i + 60;
             for (int j = 1; j <= 8; j++) {
@@ -431,12 +459,19 @@ int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *x,
   }
 
   if (isw == 4) {
+    int inpi = 0;
     for (int lz = 0; lz < 4; lz++) {
       double ti = PG4[lz];
       for (int ly = 0; ly < 4; ly++) {
+        inpi++;
         double si = PG4[ly];
         for (int lx = 0; lx < 4; lx++) {
           double ri = PG4[lx];
+          if (vp) {
+            prop_porosity(&a, &b, &c, &imod_biot, &coef_biot, &perm, vp + 7 * (inpi - 1),
+                          fluid_sw);
+            pm_d = vp[7 * (inpi - 1) + 2];
+          }
           sfhexa8(hp, hpx, hpy, hpz, ri, si, ti, 1, 1);
           if (jacob3d(hpx, hpy, hpz, xj, xji, x, &det, 8, 1)) return -1;
           w = WG4[lx] * WG4[ly] * WG4[lz] * det;
@@ -460,8 +495,8 @@ int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *x,
             P(l2) = P(l2) - wt3 * gl[1];
             P(l3) = P(l3) - wt3 * gl[2];
           }
-          wt1 = w * dt_fluid_perm;
-          wt2 = w * dt_perm;
+          wt1 = vp ? w * env->dt * fluid_d * perm : w * dt_fluid_perm;
+          wt2 = vp ? w * env->dt * perm : w * dt_perm;
           for (int i = 1; i <= 8; i++) {
             int l = i + 60;
             P(l) = P(l) + wt1 * (hpx[i - 1] * gl[0] + hpy[i - 1] * gl[1] + hpz[i - 1] * gl[2]);
@@ -615,5 +650,215 @@ int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *x,
     }
     return 0;
   }
+  return -3;
+}
+
+int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *x,
+               const double *u, const double *dp, double *p, double *s, int isw,
+               const orc_env *env) {
+  return elmt_hexa20(e, iqm, iqh, x, u, dp, p, s, isw, env, NULL);
+}
+
+/* ---- variable properties (Poromec.f) ---- */
+
+/* konzey_carman (Poromec.f:18-30) */
+double orc_konzey_carman(double perm0, double poro, double poro0) {
+  double tmp1 = (poro * poro * poro) / ((1.0 - poro) * (1.0 - poro));
+  double tmp2 = ((1.0 - poro0) * (1.0 - poro0)) / (poro0 * poro0 * poro0);
+  return perm0 * tmp1 * tmp2;
+}
+
+/* hashin_shtrikman (Poromec.f:113-163): drained K, mu, 1/M and alpha of a
+ * porous solid of moduli ks, mus at porosity poro */
+void orc_hashin_shtrikman(double poro, double ks, double mus, double *k, double *mu,
+                          double *im_biot, double *c_biot) {
+  double ha = 1.0 - poro;
+  double hb = 9.0 + 6.0 * poro;
+  double hc = 8.0 + 12.0 * poro;
+  *k = (4.0 * ks * mus * ha) / (3.0 * ks * poro + 4.0 * mus);
+  *mu = (mus * ha * (9.0 * ks + 8 * mus)) / (ks * hb + mus * hc);
+  *c_biot = 1.0 - *k / ks;
+  *im_biot = (*c_biot - poro) / ks;
+}
+
+/* ks_and_mus (Poromec.f:48-111): the solid moduli whose Hashin-Shtrikman
+ * bound at poro is the drained (k, mu); returns -1 where the reference
+ * stops (no real root, negative moduli) */
+int orc_ks_and_mus(double ym, double ps, double poro, double *k, double *mu) {
+  double ha = 1.0 - poro;
+  double hb = 9.0 + 6.0 * poro;
+  double hc = 8.0 + 12.0 * poro;
+  double a = 32.0 * ha * ha;
+  double b = 4.0 * ha * (9.0 * *k - *mu * hc - 6.0 * poro * *k);
+  double c = 3.0 * *mu * hc * poro * *k - 4.0 * *k * *mu * hb;
+  double delta = b * b - 4.0 * a * c, mus;
+  if (delta > 0) {
+    delta = sqrt(delta);
+    mus = 0.5 * (delta - b) / a;
+    if (mus < 0) return -1;
+  } else {
+    return -1;
+  }
+  double ks = 4.0 * *k * mus / (4.0 * mus * ha - 3.0 * poro * *k);
+  if (ks < 0) return -1;
+  *k = ks;
+  *mu = mus;
+  (void)ym;
+  (void)ps;
+  return 0;
+}
+
+/* extrapol_hexa20_v2 (Elmlib.f:3540-3590): the trilinear fit through the 8
+ * corner integration points (1, 4, 13, 16, 49, 52, 61, 64 of the 4x4x4
+ * rule), evaluated at the 8 vertices.  up: ndf x 64, ue: ndf x 8. */
+void orc_extrapol_hexa20(const double *up, double *ue, int ndf) {
+  static const int IP[8] = {1, 4, 13, 16, 49, 52, 61, 64};
+  const double c1 = 0.125000000000000, c2 = 0.145157042290566;
+  const double c3 = 0.168564535400043, c4 = 0.195746635145486;
+  for (int i = 0; i < ndf; i++) {
+    double f[9], a[9];
+    for (int j = 1; j <= 8; j++) f[j] = up[(IP[j - 1] - 1) * ndf + i];
+    a[1] = c1 * (f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8]);
+    a[2] = c2 * (f[2] + f[4] + f[6] + f[8] - (f[1] + f[3] + f[5] + f[7]));
+    a[3] = c2 * (f[3] + f[4] + f[7] + f[8] - (f[1] + f[2] + f[5] + f[6]));
+    a[4] = c2 * (f[5] + f[6] + f[7] + f[8] - (f[1] + f[2] + f[3] + f[4]));
+    a[5] = c3 * (f[1] + f[4] + f[5] + f[8] - (f[2] + f[3] + f[6] + f[7]));
+    a[6] = c3 * (f[1] + f[2] + f[7] + f[8] - (f[3] + f[4] + f[5] + f[6]));
+    a[7] = c3 * (f[1] + f[3] + f[6] + f[8] - (f[2] + f[4] + f[5] + f[7]));
+    a[8] = c4 * (f[2] + f[3] + f[5] + f[8] - (f[1] + f[4] + f[6] + f[7]));
+    for (int j = 0; j < 8; j++) {
+      double rnj = RN[j], snj = SN[j], tnj = TN[j];
+      ue[j * ndf + i] = a[1] + a[2] * rnj + a[3] * snj + a[4] * tnj + a[5] * rnj * snj +
+                        a[6] * snj * tnj + a[7] * rnj * tnj + a[8] * rnj * snj * tnj;
+    }
+  }
+}
+
+/* elmt18_pm (Elmlib03.f:2640-3640): the hexa20/8 Biot element with
+ * properties per integration point, vp (7 x 64).  e(1), e(2) are the
+ * solid's K and mu after initial_prop, e(3) the initial permeability, e(8)
+ * the initial porosity.  isw 2, 4: the elmt17_pm integrals with the point's
+ * properties; isw 3: stresses and flux at the points, extrapolated to the
+ * vertices; isw 7: the points' porosity extrapolated to the vertices;
+ * isw 10: porosity += alpha tr(du) + dp / M at each point (alpha, M the
+ * material's), then Kozeny-Carman permeability, mixture density and
+ * Hashin-Shtrikman moduli of the new porosity, in vp. */
+int orc_elmt18(const double *e, const int *iqm, const int *iqh, const double *x,
+               const double *u, const double *dp, double *p, double *s, double *vp, int isw,
+               const orc_env *env) {
+  if (isw == 2 || isw == 4) return elmt_hexa20(e, iqm, iqh, x, u, dp, p, s, isw, env, vp);
+  double hu[20], hux[20], huy[20], huz[20], hp[8], hpx[8], hpy[8], hpz[8];
+  double xj[3][3], xji[3][3], det, epsi[6];
+  const double scale = 1.0e-06;
+  const double div43 = 0.133333333333333e+01, div23 = 0.666666666666667;
+  double gl[3] = {env->gravity[0], env->gravity[1], env->gravity[2]};
+  double pm_d = e[5] * scale, fluid_d = e[6] * scale;
+  double fluid_sw = fluid_d * env->gravity_mod;
+  double modK = e[0], mu = e[1];
+  double perm0 = e[2], perm = perm0 / fluid_sw;
+  double imod_biot = 1.0 / e[3], coef_biot = e[4];
+  double a = modK + div43 * mu, b = (modK - div23 * mu) / a, c = mu / a;
+  double poro0 = e[7];
+  if (isw == 3) {
+    double itx[64 * 6], itxb[64 * 6], iflux[64 * 3], etx[48], etxb[48], eflux[24];
+    int inpi = 0;
+    for (int lz = 0; lz < 4; lz++) {
+      double ti = PG4[lz];
+      for (int ly = 0; ly < 4; ly++) {
+        double si = PG4[ly];
+        for (int lx = 0; lx < 4; lx++) {
+          inpi++;
+          double ri = PG4[lx];
+          prop_porosity(&a, &b, &c, &imod_biot, &coef_biot, &perm, vp + 7 * (inpi - 1),
+                        fluid_sw);
+          sfhexa8(hp, hpx, hpy, hpz, ri, si, ti, 1, 1);
+          if (jacob3d(hpx, hpy, hpz, xj, xji, x, &det, 8, 1)) return -1;
+          sfhexa20(hu, hux, huy, huz, ri, si, ti, 0, 1);
+          jacob3d(hux, huy, huz, xj, xji, x, &det, 20, 0);
+          deform3d(hux, huy, huz, u, epsi, 20);
+          double *t = itx + 6 * (inpi - 1), *tb = itxb + 6 * (inpi - 1);
+          stress3d(a, b, c, epsi, t);
+          double dpm = hp[0] * dp[0] + hp[1] * dp[1] + hp[2] * dp[2] + hp[3] * dp[3] +
+                       hp[4] * dp[4] + hp[5] * dp[5] + hp[6] * dp[6] + hp[7] * dp[7];
+          double pi = hp[0] * U(61) + hp[1] * U(62) + hp[2] * U(63) + hp[3] * U(64) +
+                      hp[4] * U(65) + hp[5] * U(66) + hp[6] * U(67) + hp[7] * U(68);
+          dpm = coef_biot * dpm;
+          double pm = coef_biot * pi;
+          t[0] = t[0] - dpm;
+          t[1] = t[1] - dpm;
+          t[2] = t[2] - dpm;
+          tb[0] = t[0] + pm;
+          tb[1] = t[1] + pm;
+          tb[2] = t[2] + pm;
+          tb[3] = t[3];
+          tb[4] = t[4];
+          tb[5] = t[5];
+          double *q = iflux + 3 * (inpi - 1);
+          q[0] = fluid_d * gl[0];
+          q[1] = fluid_d * gl[1];
+          q[2] = fluid_d * gl[2];
+          for (int k = 0; k < 8; k++) {
+            q[0] = q[0] - hpx[k] * U(61 + k);
+            q[1] = q[1] - hpy[k] * U(61 + k);
+            q[2] = q[2] - hpz[k] * U(61 + k);
+          }
+          q[0] = perm * q[0];
+          q[1] = perm * q[1];
+          q[2] = perm * q[2];
+        }
+      }
+    }
+    orc_extrapol_hexa20(iflux, eflux, 3);
+    orc_extrapol_hexa20(itx, etx, 6);
+    orc_extrapol_hexa20(itxb, etxb, 6);
+    for (int i = 1; i <= 8; i++) {
+      int tp = 6 * i - 5, tp1 = 6 * i + 43, tq = 3 * i + 94;
+      for (int k = 0; k < 6; k++) {
+        P(tp + k) = etx[6 * (i - 1) + k];
+        P(tp1 + k) = etxb[6 * (i - 1) + k];
+      }
+      for (int k = 0; k < 3; k++) P(tq + k) = eflux[3 * (i - 1) + k];
+    }
+    return 0;
+  }
+  if (isw == 7) {
+    double ip[64], pe[8];
+    for (int i = 0; i < 64; i++) ip[i] = vp[7 * i];
+    orc_extrapol_hexa20(ip, pe, 1);
+    for (int i = 0; i < 8; i++) p[i] = pe[i];
+    return 0;
+  }
+  if (isw == 10) {
+    int inpi = 0;
+    for (int lz = 0; lz < 4; lz++) {
+      double ti = PG4[lz];
+      for (int ly = 0; ly < 4; ly++) {
+        double si = PG4[ly];
+        for (int lx = 0; lx < 4; lx++) {
+          inpi++;
+          double ri = PG4[lx];
+          sfhexa8(hp, hpx, hpy, hpz, ri, si, ti, 1, 1);
+          if (jacob3d(hpx, hpy, hpz, xj, xji, x, &det, 8, 1)) return -1;
+          sfhexa20(hu, hux, huy, huz, ri, si, ti, 0, 1);
+          jacob3d(hux, huy, huz, xj, xji, x, &det, 20, 0);
+          deform3d(hux, huy, huz, u, epsi, 20);
+          double pi = hp[0] * U(61) + hp[1] * U(62) + hp[2] * U(63) + hp[3] * U(64) +
+                      hp[4] * U(65) + hp[5] * U(66) + hp[6] * U(67) + hp[7] * U(68);
+          pi = imod_biot * pi;
+          double tmp = coef_biot * (epsi[0] + epsi[1] + epsi[2]) + pi;
+          double *v = vp + 7 * (inpi - 1);
+          v[0] = v[0] + tmp;
+          double poro = v[0];
+          v[1] = orc_konzey_carman(perm0, poro, poro0);
+          v[2] = (1.0 - poro) * pm_d + poro * fluid_d;
+          orc_hashin_shtrikman(poro, modK, mu, &v[3], &v[4], &v[5], &v[6]);
+        }
+      }
+    }
+    return 0;
+  }
+  (void)iqm;
+  (void)iqh;
+  (void)s;
   return -3;
 }

@@ -9,6 +9,10 @@
  *   orc_porosity       porosity_form               Pform.f:969-1176
  *   orc_tform          tform_pm, effective_stress  Pform.f:433-700, 2487-2503
  *   orc_initial_porosity initial_porosity          Pform.f:1331-1376
+ *   orc_initial_prop   initial_prop                Pform.f:1434-1540
+ *   orc_update_prop    update_prop (isw 10)        Pform.f:1176-1330
+ * Elements of materials of type 18 go to elmt18_pm with their points'
+ * properties (elmlib_pm, Elmlib.f:77-130).
  *   orc_pload          pload_pm                    Pload.f:1-92
  *   orc_delta_predict  delta_predict_pm            Predict.f:117-158
  *   orc_delta_update   delta_update_pm             Predict.f:159-215
@@ -73,6 +77,19 @@ static void gather(const orc_model *md, int nel, int *ld, double *ul, double *dp
   }
 }
 
+static int elem_type(const orc_model *md, int ma) { return md->ie ? md->ie[ma - 1] : 17; }
+
+/* elmlib_pm: element nel of material ma */
+static int elmlib(orc_model *md, int nel, int ma, const int *iqm, const int *iqh,
+                  const double *xl, const double *ul, const double *dpl, double *pl, double *sl,
+                  int isw, const orc_env *env) {
+  const double *el = md->e + (size_t)(ma - 1) * ORC_PROP;
+  if (elem_type(md, ma) == 18)
+    return orc_elmt18(el, iqm, iqh, xl, ul, dpl, pl, sl, md->vpropel + (size_t)nel * 448, isw,
+                      env);
+  return orc_elmt17(el, iqm, iqh, xl, ul, dpl, pl, sl, isw, env);
+}
+
 int orc_pform(orc_model *md, int isw, int lhs, int rhs, double *bvec, const orc_env *env) {
   int ld[NST];
   double ul[NST], pl[NST], dpl[ORC_NENV], xl[3 * ORC_NENV];
@@ -85,9 +102,8 @@ int orc_pform(orc_model *md, int isw, int lhs, int rhs, double *bvec, const orc_
   for (int nel = 0; nel < md->numel && rc == 0; nel++) {
     gather(md, nel, ld, ul, dpl, xl, pl);
     int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
-    const double *el = md->e + (size_t)(ma - 1) * ORC_PROP;
-    rc = orc_elmt17(el, md->eload + 7 * (size_t)nel, md->eloadp + 7 * (size_t)nel, xl, ul, dpl,
-                    pl, sl, isw, env);
+    rc = elmlib(md, nel, ma, md->eload + 7 * (size_t)nel, md->eloadp + 7 * (size_t)nel, xl, ul,
+                dpl, pl, sl, isw, env);
     if (rc) break;
     assemble_csr(sl, pl, ld, md->ia, md->ja, md->al, md->ad, bvec, md->neq, lhs, rhs);
   }
@@ -130,8 +146,7 @@ int orc_tform(orc_model *md, double *tx, double *txe, double *txb, double *flux,
     gather(md, nel, ld, ul, dpl, xl, pl);
     for (int i = 0; i < 8 * 15; i++) p[i] = 0.0;
     int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
-    const double *el = md->e + (size_t)(ma - 1) * ORC_PROP;
-    rc = orc_elmt17(el, iqz, iqz, xl, ul, dpl, p, NULL, 3, env);
+    rc = elmlib(md, nel, ma, iqz, iqz, xl, ul, dpl, p, NULL, 3, env);
     if (rc) break;
     for (int i = 0; i < ORC_NENV; i++) {
       int no = md->ix[(size_t)nel * NEN1 + i];
@@ -174,8 +189,7 @@ void orc_porosity(orc_model *md, double *dporo, int *ic, const orc_env *env) {
     gather(md, nel, ld, ul, NULL, xl, pl);
     for (int i = 0; i < ORC_NENV; i++) pl[i] = 0.0;
     int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
-    const double *el = md->e + (size_t)(ma - 1) * ORC_PROP;
-    orc_elmt17(el, iqz, iqz, xl, ul, NULL, pl, NULL, 7, env);
+    elmlib(md, nel, ma, iqz, iqz, xl, ul, NULL, pl, NULL, 7, env);
     for (int i = 0; i < ORC_NENV; i++) {
       int no = md->ix[(size_t)nel * NEN1 + i];
       if (no <= 0) continue;
@@ -183,8 +197,76 @@ void orc_porosity(orc_model *md, double *dporo, int *ic, const orc_env *env) {
       dporo[no - 1] = dporo[no - 1] + pl[i];
     }
   }
+  /* with variable properties the elements return the porosity itself
+   * (elmt18_pm isw 7), otherwise its change */
   for (int i = 0; i < md->nnode; i++)
-    if (md->fnno[i] == 1) md->porosity[i] = md->porosity[i] + dporo[i] / ic[i];
+    if (md->fnno[i] == 1) {
+      if (md->vprop[0]) md->porosity[i] = dporo[i] / ic[i];
+      else md->porosity[i] = md->porosity[i] + dporo[i] / ic[i];
+    }
+}
+
+int orc_update_prop(orc_model *md, const orc_env *env) {
+  int ld[NST];
+  double ul[NST], pl[NST], xl[3 * ORC_NENV], vpl[448];
+  int iqz[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int nel = 0; nel < md->numel; nel++) {
+    gather(md, nel, ld, ul, NULL, xl, pl);
+    int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
+    if (elem_type(md, ma) != 18) continue;  /* elmt17_pm has no isw 10 */
+    double *vp = md->vpropel + (size_t)nel * 448;
+    memcpy(vpl, vp, sizeof(vpl));
+    int rc = orc_elmt18(md->e + (size_t)(ma - 1) * ORC_PROP, iqz, iqz, xl, ul, NULL, pl, NULL,
+                        vpl, 10, env);
+    if (rc) return rc;
+    /* the porosity always, the rest as the conseq flags ask */
+    for (int j = 0; j < 64; j++) {
+      vp[7 * j] = vpl[7 * j];
+      if (md->vprop[1]) vp[7 * j + 1] = vpl[7 * j + 1];
+      if (md->vprop[2]) vp[7 * j + 2] = vpl[7 * j + 2];
+      if (md->vprop[3])
+        for (int k = 3; k < 7; k++) vp[7 * j + k] = vpl[7 * j + k];
+    }
+  }
+  return 0;
+}
+
+int orc_initial_prop(orc_model *md, double *e, int numat) {
+  const double scale = 1.0e-06;
+  for (int ma = 0; ma < numat; ma++) {
+    double *em = e + (size_t)ma * ORC_PROP;
+    double ym = em[0], ps = em[1], ro = em[5], rof = em[6], poro = em[7];
+    double lam = ym * ps / ((1.0 + ps) * (1.0 - 2.0 * ps));
+    double mu = (0.5 * ym) / (1.0 + ps);
+    double k = lam + (2.0 / 3.0) * mu;
+    if (md->vprop[2]) em[5] = (ro - poro * rof) / (1.0 - poro);
+    if (md->vprop[3] && orc_ks_and_mus(ym, ps, poro, &k, &mu)) return -1;
+    em[0] = k;
+    em[1] = mu;
+  }
+  for (int nel = 0; nel < md->numel; nel++) {
+    int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
+    const double *em = e + (size_t)(ma - 1) * ORC_PROP;
+    double k = em[0], mu = em[1], perm = em[2], ro = em[5], rof = em[6], poro = em[7];
+    double ibiot = 1 / em[3], cbiot = em[4];
+    double *vp = md->vpropel + (size_t)nel * 448;
+    for (int i = 0; i < 64; i++) {
+      double *v = vp + 7 * i;
+      v[0] = poro;
+      v[1] = perm;
+      v[2] = md->vprop[2] ? (1.0 - poro) * ro + poro * rof : ro;
+      v[2] = v[2] * scale;
+      if (md->vprop[3]) {
+        orc_hashin_shtrikman(poro, k, mu, &v[3], &v[4], &v[5], &v[6]);
+      } else {
+        v[3] = k;
+        v[4] = mu;
+        v[5] = ibiot;
+        v[6] = cbiot;
+      }
+    }
+  }
+  return 0;
 }
 
 void orc_pload(orc_model *md, const orc_env *env) {
@@ -335,6 +417,10 @@ int orc_step(orc_model *md, const orc_cfg *cfg, orc_env *env, orc_step_info *inf
     }
     i++;
     lhs = 0;
+  }
+  if (md->vprop[0]) {  /* Main.f:1070-1079 */
+    rc = orc_update_prop(md, env);
+    if (rc) return rc;
   }
   double *dporo = malloc(sizeof(double) * (size_t)md->nnode);
   int *ic = malloc(sizeof(int) * (size_t)md->nnode);

@@ -59,6 +59,12 @@ typedef struct {
   double *dp;         /* nnode */
   double *porosity;   /* nnode */
   double *b0, *b, *x0, *m; /* neq */
+  /* variable properties (elmt18_pm): element type of each material (17 or
+   * 18; NULL: all 17), the conseq flags vprop(1..4) and the 7 x 64
+   * properties of every element's integration points */
+  const int *ie;      /* numat */
+  int vprop[4];
+  double *vpropel;    /* 7 x 64 x numel */
 } orc_model;
 
 typedef struct {
@@ -82,6 +88,16 @@ typedef struct {
 int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *xl,
                const double *ul, const double *dpl, double *p, double *s, int isw,
                const orc_env *env);
+/* elmt18_pm (Elmlib03.f:2640-3640), isw 2, 3, 4, 7, 10, with vp the 7 x 64
+ * properties of the element's integration points (isw 10 updates them) */
+int orc_elmt18(const double *e, const int *iqm, const int *iqh, const double *xl,
+               const double *ul, const double *dpl, double *p, double *s, double *vp, int isw,
+               const orc_env *env);
+double orc_konzey_carman(double perm0, double poro, double poro0);
+void orc_hashin_shtrikman(double poro, double ks, double mus, double *k, double *mu,
+                          double *im_biot, double *c_biot);
+int orc_ks_and_mus(double ym, double ps, double poro, double *k, double *mu);
+void orc_extrapol_hexa20(const double *up, double *ue, int ndf);
 double orc_tload(int nc, double t, const double *x, double u, double *vc, const orc_env *env);
 
 /* ---- structure (Adjacency.f, Rdata.f, Graph.f, Reord.f, Numeq.f, Csr.f) ---- */
@@ -101,6 +117,13 @@ int orc_pform(orc_model *md, int isw, int lhs, int rhs, double *bvec, const orc_
 void orc_porosity(orc_model *md, double *dporo, int *ic, const orc_env *env);
 /* initial_porosity (Pform.f:1331-1376): vertex average of e(8) */
 void orc_initial_porosity(orc_model *md);
+/* initial_prop (Pform.f:1434-1540): the solid's moduli and density into
+ * e(1), e(2), e(6) of every material (e is numat x PROP, changed in place)
+ * and the initial properties of every integration point; -1 where
+ * ks_and_mus stops */
+int orc_initial_prop(orc_model *md, double *e, int numat);
+/* update_prop (Pform.f:1176-1330) with isw 10 after the nonlinear loop */
+int orc_update_prop(orc_model *md, const orc_env *env);
 /* tform_pm (Pform.f:433-700) + effective_stress (Pform.f:2487-2503): vertex
  * averages of total stress tx (6 x nnode), Terzaghi tx + p (txe), Biot's
  * tx + alpha p (txb) and the Darcy flux (3 x nnode); other nodes get zero. */
