@@ -133,6 +133,23 @@ int pm_set_materials(pm_ctx *ctx, int numat, const double *e, const int *eload,
                      const int *eloadp, const int *load, const double *fload,
                      const double *gravity, double gravity_mod);
 
+/* Variable properties at the integration points (elmt18_pm,
+ * Elmlib03.f:2640-3640): ie (numat) gives each material's element type, 17
+ * or 18 (elmt17_pm / elmt18_pm), and vprop[4] the conseq macro's flags
+ * (read_constitutive_equation, Rdata.f:2545-2640): vprop, kozeny_carman,
+ * density, hashin_shtrikman.  Like check_element (Rdata.f:3703-3760) it
+ * requires an elmt18 material exactly when vprop is on.  It then runs
+ * initial_prop (Pform.f:1434-1540), which turns e(1), e(2) into the
+ * solid's K and mu (e(6) into its density) and sets the 7 properties of
+ * every element's 64 points; every step then runs update_prop
+ * (Pform.f:1176-1330) after the nonlinear loop, and the nodal porosity
+ * becomes the points' porosity extrapolated to the vertices.  Call after
+ * pm_set_materials (which resets it); PM_ERR_ARG where ks_and_mus has no
+ * solution, PM_ERR_UNSUPPORTED in the batching form. */
+int pm_set_vprop(pm_ctx *ctx, const int *ie, const int *vprop);
+/* The points' properties (7, 64, numel) in the reference's vpropel layout. */
+int pm_get_vprop(pm_ctx *ctx, double *vpropel);
+
 /* Nodal loads and prescribed values: nload (4, nnode) time-function ids and
  * f (4, nnode) values, as pload_pm reads them (Pload.f:1-92).  Optional. */
 int pm_set_nodal_loads(pm_ctx *ctx, const int *nload, const double *f);

@@ -89,6 +89,8 @@ def lib():
     L.pm_set_vectors.argtypes = [_vp, _vp, _vp, _vp]
     L.pm_step.argtypes = [_vp, C.c_double, C.POINTER(SolverCfg), C.POINTER(StepInfo)]
     L.pm_tform.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.pm_set_vprop.argtypes = [_vp, _vp, _vp]
+    L.pm_get_vprop.argtypes = [_vp, _vp]
     L.pm_batch_define.argtypes = [_vp, C.c_int, _vp]
     L.pm_batch_iters.argtypes = [_vp, _vp]
     L.pm_batch_graphs.argtypes = [_vp, C.c_int]

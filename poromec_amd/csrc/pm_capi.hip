@@ -66,6 +66,8 @@ pm_ctx::~pm_ctx() {
   FREE(d_tf_ptr);
   FREE(d_tf_src);
   FREE(d_tf_out);
+  FREE(d_ie);
+  FREE(d_vp);
   if (batch) pm_batch_free(batch);
   FREE(d_eq_box);
   free_matrix(A);
@@ -464,6 +466,8 @@ extern "C" int pm_set_materials(pm_ctx *c, int numat, const double *e, const int
     }
   c->numat = numat;
   c->h_load = ld;
+  c->h_e.assign(e, e + (size_t)numat * PM_PROP);
+  c->vprop[0] = c->vprop[1] = c->vprop[2] = c->vprop[3] = 0;  // pm_set_vprop again
   PM_TRY(dupload(c, &c->d_e, e, (size_t)numat * PM_PROP));
   PM_TRY(dupload(c, &c->d_eload, el14.data(), el14.size()));
   PM_TRY(dupload(c, &c->d_load, ld.data(), ld.size()));
@@ -735,6 +739,111 @@ extern "C" int pm_step(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_i
   return pm_step_dev(c, dt, cfg, info ? info : &local);
 }
 
+// ks_and_mus (Poromec.f:48-111): the solid moduli whose Hashin-Shtrikman
+// bound at poro is the drained (k, mu); false where the reference stops
+static bool ks_and_mus(double poro, double &k, double &mu) {
+  const double ha = 1.0 - poro;
+  const double hb = 9.0 + 6.0 * poro;
+  const double hc = 8.0 + 12.0 * poro;
+  const double a = 32.0 * ha * ha;
+  const double b = 4.0 * ha * (9.0 * k - mu * hc - 6.0 * poro * k);
+  const double cc = 3.0 * mu * hc * poro * k - 4.0 * k * mu * hb;
+  double delta = b * b - 4.0 * a * cc;
+  if (!(delta > 0)) return false;
+  delta = std::sqrt(delta);
+  const double mus = 0.5 * (delta - b) / a;
+  if (mus < 0) return false;
+  const double ks = 4.0 * k * mus / (4.0 * mus * ha - 3.0 * poro * k);
+  if (ks < 0) return false;
+  k = ks;
+  mu = mus;
+  return true;
+}
+
+static void hashin_shtrikman_h(double poro, double ks, double mus, double *v) {
+  const double ha = 1.0 - poro;
+  const double hb = 9.0 + 6.0 * poro;
+  const double hc = 8.0 + 12.0 * poro;
+  v[0] = (4.0 * ks * mus * ha) / (3.0 * ks * poro + 4.0 * mus);
+  v[1] = (mus * ha * (9.0 * ks + 8 * mus)) / (ks * hb + mus * hc);
+  v[3] = 1.0 - v[0] / ks;
+  v[2] = (v[3] - poro) / ks;
+}
+
+extern "C" int pm_set_vprop(pm_ctx *c, const int *ie, const int *vprop) {
+  if (!c || !ie) return PM_ERR_ARG;
+  if (!c->have_materials) return PM_ERR_STATE;
+  if (c->batch) return PM_ERR_UNSUPPORTED;
+  PM_CHECK_HIP(hipSetDevice(c->device));
+  const int numat = c->numat;
+  int f[4] = {0, 0, 0, 0};
+  for (int i = 0; i < 4 && vprop; i++) f[i] = vprop[i] != 0;
+  if (f[1] || f[2] || f[3]) f[0] = 1;  // read_constitutive_equation, Rdata.f:2630-2633
+  bool any18 = false;
+  for (int ma = 0; ma < numat; ma++) {
+    if (ie[ma] != 17 && ie[ma] != 18) return PM_ERR_UNSUPPORTED;
+    any18 |= ie[ma] == 18;
+  }
+  if (f[0] != (int)any18) return PM_ERR_ARG;  // check_element, Rdata.f:3744-3758
+  for (int i = 0; i < 4; i++) c->vprop[i] = 0;
+  if (!f[0]) {
+    PM_TRY(dupload(c, &c->d_e, c->h_e.data(), c->h_e.size()));
+    return PM_OK;
+  }
+  // initial_prop (Pform.f:1434-1540), from the materials as given
+  std::vector<double> e = c->h_e;
+  for (int ma = 0; ma < numat; ma++) {
+    double *em = e.data() + (size_t)ma * PM_PROP;
+    const double ym = em[0], ps = em[1], ro = em[5], rof = em[6], poro = em[7];
+    const double lam = ym * ps / ((1.0 + ps) * (1.0 - 2.0 * ps));
+    double mu = (0.5 * ym) / (1.0 + ps);
+    double k = lam + (2.0 / 3.0) * mu;
+    if (f[2]) em[5] = (ro - poro * rof) / (1.0 - poro);
+    if (f[3] && !ks_and_mus(poro, k, mu)) return PM_ERR_ARG;
+    em[0] = k;
+    em[1] = mu;
+  }
+  const PmHostMesh &m = c->mesh;
+  std::vector<double> vp((size_t)m.numel * 64 * 7);
+  for (int el = 0; el < m.numel; el++) {
+    const int ma = m.ix20[(size_t)el * 21 + 20];
+    const double *em = e.data() + (size_t)(ma - 1) * PM_PROP;
+    const double k = em[0], mu = em[1], perm = em[2], ro = em[5], rof = em[6], poro = em[7];
+    const double ibiot = 1 / em[3], cbiot = em[4];
+    for (int i = 0; i < 64; i++) {
+      double *v = vp.data() + ((size_t)el * 64 + i) * 7;
+      v[0] = poro;
+      v[1] = perm;
+      v[2] = f[2] ? (1.0 - poro) * ro + poro * rof : ro;
+      v[2] = v[2] * 1.0e-06;
+      if (f[3]) {
+        hashin_shtrikman_h(poro, k, mu, v + 3);
+      } else {
+        v[3] = k;
+        v[4] = mu;
+        v[5] = ibiot;
+        v[6] = cbiot;
+      }
+    }
+  }
+  PM_TRY(dupload(c, &c->d_e, e.data(), e.size()));
+  PM_TRY(dupload(c, &c->d_vp, vp.data(), vp.size()));
+  PM_TRY(dupload(c, &c->d_ie, ie, (size_t)numat));
+  PM_CHECK_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 4; i++) c->vprop[i] = f[i];
+  return PM_OK;
+}
+
+extern "C" int pm_get_vprop(pm_ctx *c, double *vpropel) {
+  if (!c || !vpropel) return PM_ERR_ARG;
+  if (!c->vprop[0]) return PM_ERR_STATE;
+  PM_CHECK_HIP(hipSetDevice(c->device));
+  PM_CHECK_HIP(hipMemcpyAsync(vpropel, c->d_vp, sizeof(double) * (size_t)c->mesh.numel * 64 * 7,
+                              hipMemcpyDeviceToHost, c->stream));
+  PM_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return PM_OK;
+}
+
 extern "C" int pm_tform(pm_ctx *c, double *tx, double *txe, double *txb, double *flux) {
   if (!c) return PM_ERR_ARG;
   PM_CHECK_HIP(hipSetDevice(c->device));
@@ -751,6 +860,7 @@ extern "C" int pm_adjoint(pm_ctx *c, int nsteps, double dt, const pm_solver_cfg 
                           const int *sensor_dof, const double *target, int ckpt_every, double *J,
                           double *grad_logE, double *grad_logk) {
   if (!c || !cfg) return PM_ERR_ARG;
+  if (c->vprop[0]) return PM_ERR_UNSUPPORTED;  // the adjoint is of elmt17_pm's march
   PM_CHECK_HIP(hipSetDevice(c->device));
   return pm_adjoint_dev(c, nsteps, dt, cfg, nsens, sensor_dof, target, ckpt_every, J, grad_logE,
                         grad_logk);
@@ -889,6 +999,7 @@ extern "C" int pm_bench_copy(pm_ctx *c, int64_t bytes, int reps, double *ms_per_
 
 extern "C" int pm_batch_define(pm_ctx *c, int nbox, const int *elem_off) {
   if (!c || !elem_off) return PM_ERR_ARG;
+  if (c->vprop[0]) return PM_ERR_UNSUPPORTED;
   PM_CHECK_HIP(hipSetDevice(c->device));
   return pm_batch_define_dev(c, nbox, elem_off);
 }

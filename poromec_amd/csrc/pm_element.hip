@@ -48,7 +48,15 @@ struct ElemArgs {
   // reference summation order: element arrays are stored, not scattered
   double *ke;          // (numel, 68, 68) element matrices, or null
   double *pe;          // (numel, 68) element vectors, or null
+  // variable properties: element type per material (null: all 17) and the
+  // 7 properties of each element's 64 integration points (elmt18_pm)
+  const int *ie;
+  const double *vp;    // (numel, 64, 7)
 };
+
+__device__ __forceinline__ bool is_elmt18(const ElemArgs &A, int el) {
+  return A.ie && A.ie[A.mat[el]] == 18;
+}
 
 #define GP_CHUNK 16
 
@@ -65,6 +73,7 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw2(ElemArgs A) {
   __shared__ double s_hp[GP_CHUNK][8][4];   // value + global derivatives of 8 p nodes
   __shared__ double s_K[PM_NST][PM_NST + 1];
   __shared__ double s_mat[PM_PROP];
+  __shared__ double s_gp[GP_CHUNK][6];  // elmt18_pm: a, b, c, alpha, 1/M, perm per point
 
   const int tid = threadIdx.x;
   const int el = A.elems[blockIdx.x];
@@ -82,6 +91,7 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw2(ElemArgs A) {
   if (tid < PM_PROP) s_mat[tid] = A.e[(size_t)A.mat[el] * PM_PROP + tid];
   __syncthreads();
   const ElemMat M = elem_material(s_mat, A.dt, A.gmod);
+  const bool v18 = is_elmt18(A, el);
 
   // work split: Kuu node pairs p0 = tid, p1 = tid + 256 (< 400);
   // Kpu (vertex i, node j) pair tid < 160; Kpp entry tid - 160 in [0, 64)
@@ -107,6 +117,16 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw2(ElemArgs A) {
       for (int a = 0; a < 3; a++)
         for (int j = 0; j < 3; j++) s_xji[tid][3 * a + j] = xji[a][j];
       s_w[tid] = c_wg4[lx] * c_wg4[ly] * c_wg4[lz] * d;
+      if (v18) {  // prop_porosity of point lz * 16 + tid (Elmlib03.f:2895)
+        const GpProp P =
+            prop_porosity(A.vp + ((size_t)el * 64 + lz * 16 + tid) * 7, M.fluid_d * A.gmod);
+        s_gp[tid][0] = P.a;
+        s_gp[tid][1] = P.b;
+        s_gp[tid][2] = P.c;
+        s_gp[tid][3] = P.cbiot;
+        s_gp[tid][4] = P.ibiot;
+        s_gp[tid][5] = P.perm;
+      }
     }
     __syncthreads();
     for (int q = tid; q < GP_CHUNK * 28; q += PM_WG) {
@@ -135,11 +155,12 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw2(ElemArgs A) {
     __syncthreads();
     for (int g = 0; g < GP_CHUNK; g++) {
       const double w = s_w[g];
-      const double wt1 = w * wt_a;
+      const double wt1 = w * (v18 ? s_gp[g][0] : wt_a);
+      const double gb = v18 ? s_gp[g][1] : M.b, gc = v18 ? s_gp[g][2] : M.c;
       {
         const double xi = s_hu[g][pa0][0], yi = s_hu[g][pa0][1], zi = s_hu[g][pa0][2];
         const double xj = s_hu[g][pb0][0], yj = s_hu[g][pb0][1], zj = s_hu[g][pb0][2];
-        const double b = M.b, c = M.c;
+        const double b = gb, c = gc;
         k0[0] += (xi * xj + c * (yi * yj + zi * zj)) * wt1;
         k0[1] += (b * xi * yj + c * yi * xj) * wt1;
         k0[2] += (b * xi * zj + c * zi * xj) * wt1;
@@ -153,7 +174,7 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw2(ElemArgs A) {
       if (has1) {
         const double xi = s_hu[g][pa1][0], yi = s_hu[g][pa1][1], zi = s_hu[g][pa1][2];
         const double xj = s_hu[g][pb1][0], yj = s_hu[g][pb1][1], zj = s_hu[g][pb1][2];
-        const double b = M.b, c = M.c;
+        const double b = gb, c = gc;
         k1[0] += (xi * xj + c * (yi * yj + zi * zj)) * wt1;
         k1[1] += (b * xi * yj + c * yi * xj) * wt1;
         k1[2] += (b * xi * zj + c * zi * xj) * wt1;
@@ -165,13 +186,15 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw2(ElemArgs A) {
         k1[8] += (zi * zj + c * (yi * yj + xi * xj)) * wt1;
       }
       if (hasq) {
-        const double wq = s_hp[g][qa][0] * (w * M.coef_biot);
+        const double wq = s_hp[g][qa][0] * (w * (v18 ? s_gp[g][3] : M.coef_biot));
         kp[0] -= s_hu[g][qb][0] * wq;
         kp[1] -= s_hu[g][qb][1] * wq;
         kp[2] -= s_hu[g][qb][2] * wq;
       }
       if (hasr) {
-        const double w1 = w * M.imod_biot, w2 = w * M.dt_perm;
+        // elmt18_pm weights Kpp by w dt perm (Elmlib03.f:3031)
+        const double w1 = w * (v18 ? s_gp[g][4] : M.imod_biot);
+        const double w2 = v18 ? w * A.dt * s_gp[g][5] : w * M.dt_perm;
         kpp = kpp - w1 * s_hp[g][ra][0] * s_hp[g][rb][0] -
               w2 * (s_hp[g][ra][1] * s_hp[g][rb][1] + s_hp[g][ra][2] * s_hp[g][rb][2] +
                     s_hp[g][ra][3] * s_hp[g][rb][3]);
@@ -437,6 +460,16 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw4(ElemArgs A) {
   const double det = jacobian(s_xv, r, s, t, xji);
   if (det <= 1.0e-14 && sub == 0) atomicExch(A.err, PM_ERR_JACOBIAN);
   const double w = c_wg4[lx] * c_wg4[ly] * c_wg4[lz] * det;
+  // elmt18_pm: the point's properties, where the point index advances once
+  // per ly (Elmlib03.f:3171-3189), so points read those of 1..16, 4 each
+  const bool v18 = is_elmt18(A, el);
+  GpProp G{M.a, M.b, M.c, M.imod_biot, M.coef_biot, 0.0};
+  double gpm_d = M.pm_d;
+  if (v18) {
+    const double *vq = A.vp + ((size_t)el * 64 + lz * 4 + ly) * 7;
+    G = prop_porosity(vq, M.fluid_d * A.gmod);
+    gpm_d = vq[2];
+  }
   if (sub < 3) {
     // strain of the 20-node field (deform3d), stress (stress3d), -alpha dp
     double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0, wx = 0, wy = 0, wz = 0;
@@ -457,19 +490,19 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw4(ElemArgs A) {
     }
     const double e0 = ux, e1 = vy, e2 = wz, e3 = uy + vx, e4 = vz + wy, e5 = uz + wx;
     double tx[6];
-    tx[0] = (e0 + M.b * e1 + M.b * e2) * M.a;
-    tx[1] = (M.b * e0 + e1 + M.b * e2) * M.a;
-    tx[2] = (M.b * e0 + M.b * e1 + e2) * M.a;
-    tx[3] = e3 * M.c * M.a;
-    tx[4] = e4 * M.c * M.a;
-    tx[5] = e5 * M.c * M.a;
+    tx[0] = (e0 + G.b * e1 + G.b * e2) * G.a;
+    tx[1] = (G.b * e0 + e1 + G.b * e2) * G.a;
+    tx[2] = (G.b * e0 + G.b * e1 + e2) * G.a;
+    tx[3] = e3 * G.c * G.a;
+    tx[4] = e4 * G.c * G.a;
+    tx[5] = e5 * G.c * G.a;
     double dpm = 0.0;
     for (int k = 0; k < 8; k++) dpm = dpm + n8(k, r, s, t) * s_dp[k];
-    dpm = M.coef_biot * dpm;
+    dpm = G.cbiot * dpm;
     tx[0] -= dpm;
     tx[1] -= dpm;
     tx[2] -= dpm;
-    const double wt2 = w * M.pm_d;
+    const double wt2 = w * gpm_d;
     const int k0 = 7 * sub, k1 = min(20, k0 + 7);
     for (int k = k0; k < k1; k++) {
       const double wt3 = hv[k] * wt2;
@@ -487,7 +520,9 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw4(ElemArgs A) {
       dn8(k, r, s, t, dr, ds, dt);
       to_global(xji, dr, ds, dt, gx[k], gy[k], gz[k]);
     }
-    const double wt1 = w * M.dt_fluid_perm, wt2 = w * M.dt_perm;
+    // elmt18_pm: w dt rho_f perm and w dt perm (Elmlib03.f:3242-3243)
+    const double wt1 = v18 ? w * A.dt * M.fluid_d * G.perm : w * M.dt_fluid_perm;
+    const double wt2 = v18 ? w * A.dt * G.perm : w * M.dt_perm;
     for (int i = 0; i < 8; i++) {
       double tmp = 0.0;
       for (int j = 0; j < 8; j++)
@@ -538,6 +573,15 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw7(ElemArgs A, double *dporo, 
   for (int k = 0; k < 8; k++)
     for (int d = 0; d < 3; d++) xv[k][d] = A.xv[(size_t)nodes[k] * 3 + d];
   const double *e = A.e + (size_t)A.mat[el] * PM_PROP;
+  const int no = nodes[v];
+  if (is_elmt18(A, el)) {  // the points' porosity extrapolated (Elmlib03.f:3527-3538)
+    double f[8];
+    for (int j = 0; j < 8; j++) f[j] = A.vp[((size_t)el * 64 + c_gp_corner[j]) * 7];
+    const double val = extrapol_vertex(f, v);
+    if (pv) pv[(size_t)el * 8 + v] = val;
+    else dporo[no] += val;
+    return;
+  }
   const double r = c_rn[v], s = c_sn[v], t = c_tn[v];
   double xji[3][3];
   const double det = jacobian(xv, r, s, t, xji);
@@ -552,10 +596,53 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw7(ElemArgs A, double *dporo, 
     vy = vy + gy * A.u[(size_t)no * 4 + 1];
     wz = wz + gz * A.u[(size_t)no * 4 + 2];
   }
-  const int no = nodes[v];
   const double val = e[4] * (ux + vy + wz) + (1.0 / e[3]) * A.u[(size_t)no * 4 + 3];
   if (pv) pv[(size_t)el * 8 + v] = val;  // gathered per node in element order
   else dporo[no] += val;  // colour order: no two elements of a colour share a vertex
+}
+
+// ---------------------------------------------------------------------------
+// isw = 10 of elmt18_pm (Elmlib03.f:3558-3602) inside update_prop
+// (Pform.f:1176-1330): one thread per integration point.  u holds the
+// step's increments here.  The porosity gains alpha tr(du) + dp / M (the
+// material's alpha and M); the permeability (Kozeny-Carman), density and
+// Hashin-Shtrikman moduli of the new porosity are kept as the flags ask.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_elmt_isw10(ElemArgs A, int numel, int f2, int f3, int f4,
+                                                    double *vp) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int el = (int)(gid >> 6), q = (int)(gid & 63);
+  if (el >= numel || !is_elmt18(A, el)) return;
+  const int lx = q & 3, ly = (q >> 2) & 3, lz = q >> 4;
+  const int *nodes = A.ix + (size_t)el * 20;
+  double xv[8][3];
+  for (int k = 0; k < 8; k++)
+    for (int d = 0; d < 3; d++) xv[k][d] = A.xv[(size_t)nodes[k] * 3 + d];
+  const double *e = A.e + (size_t)A.mat[el] * PM_PROP;
+  const double r = c_pg4[lx], s = c_pg4[ly], t = c_pg4[lz];
+  double xji[3][3];
+  const double det = jacobian(xv, r, s, t, xji);
+  if (det <= 1.0e-14) atomicExch(A.err, PM_ERR_JACOBIAN);
+  double ux = 0, vy = 0, wz = 0;
+  for (int k = 0; k < 20; k++) {
+    double h, dr, ds, dt, gx, gy, gz;
+    dn20(k, r, s, t, h, dr, ds, dt);
+    to_global(xji, dr, ds, dt, gx, gy, gz);
+    const double *uk = A.u + (size_t)nodes[k] * 4;
+    ux = ux + gx * uk[0];
+    vy = vy + gy * uk[1];
+    wz = wz + gz * uk[2];
+  }
+  double pi = 0.0;
+  for (int k = 0; k < 8; k++) pi = pi + n8(k, r, s, t) * A.u[(size_t)nodes[k] * 4 + 3];
+  pi = (1.0 / e[3]) * pi;
+  const double tmp = e[4] * (ux + vy + wz) + pi;
+  double *v = vp + ((size_t)el * 64 + q) * 7;
+  const double poro = v[0] + tmp;
+  v[0] = poro;
+  if (f2) v[1] = konzey_carman(e[2], poro, e[7]);
+  if (f3) v[2] = (1.0 - poro) * (e[5] * 1.0e-06) + poro * (e[6] * 1.0e-06);
+  if (f4) hashin_shtrikman(poro, e[0], e[1], v[3], v[4], v[5], v[6]);
 }
 
 // ---------------------------------------------------------------------------
@@ -619,6 +706,8 @@ static ElemArgs elem_args(pm_ctx *c, double dt, double t, double *bvec, int lhs,
   A.rhs = rhs;
   A.ke = nullptr;
   A.pe = nullptr;
+  A.ie = c->vprop[0] ? c->d_ie : nullptr;
+  A.vp = c->d_vp;
   return A;
 }
 
@@ -710,9 +799,12 @@ int pm_pform_dev(pm_ctx *c, int isw, int lhs, int rhs, double dt, double t, doub
   return PM_OK;
 }
 
-__global__ void k_poro_update(int nnodev, const int *ic, const double *dporo, double *poro) {
+// with variable properties the elements give the porosity itself, not its
+// change (porosity_form, Pform.f:1158-1172)
+__global__ void k_poro_update(int nnodev, const int *ic, const double *dporo, double *poro,
+                              int absolute) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nnodev) poro[i] = poro[i] + dporo[i] / ic[i];
+  if (i < nnodev) poro[i] = absolute ? dporo[i] / ic[i] : poro[i] + dporo[i] / ic[i];
 }
 
 int pm_porosity_dev(pm_ctx *c, double dt) {
@@ -739,7 +831,18 @@ int pm_porosity_dev(pm_ctx *c, double dt) {
   }
   int nv = c->mesh.nnodev;
   hipLaunchKernelGGL(k_poro_update, dim3((nv + 255) / 256), dim3(256), 0, c->stream, nv, c->d_ic,
-                     c->d_dporo, c->d_poro);
+                     c->d_dporo, c->d_poro, c->vprop[0]);
+  PM_CHECK_HIP(hipGetLastError());
+  return read_elem_err(c);
+}
+
+int pm_update_prop_dev(pm_ctx *c, double dt) {
+  if (!c->vprop[0]) return PM_OK;
+  PM_CHECK_HIP(hipMemsetAsync(c->d_elem_err, 0, sizeof(int), c->stream));
+  ElemArgs A = elem_args(c, dt, c->t, nullptr, 0, 0);
+  const int64_t nthreads = (int64_t)c->mesh.numel * 64;
+  hipLaunchKernelGGL(k_elmt_isw10, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0,
+                     c->stream, A, c->mesh.numel, c->vprop[1], c->vprop[2], c->vprop[3], c->d_vp);
   PM_CHECK_HIP(hipGetLastError());
   return read_elem_err(c);
 }

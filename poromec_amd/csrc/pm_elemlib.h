@@ -226,3 +226,67 @@ __device__ __forceinline__ void to_global(const double xji[3][3], double a, doub
   gz = xji[2][0] * a + xji[2][1] * b + xji[2][2] * c;
 }
 
+
+// ---------------------------------------------------------------------------
+// variable properties (elmt18_pm, Elmlib03.f:2640-3640; Poromec.f)
+// ---------------------------------------------------------------------------
+// prop_porosity (Poromec.f:165-200): one integration point's D-matrix
+// constants, Biot moduli and permeability from its properties
+// vp[0..6] = porosity, permeability, density, K, mu, 1/M, alpha; div43 and
+// div23 are the reference's 15-digit constants
+struct GpProp {
+  double a, b, c, ibiot, cbiot, perm;
+};
+__device__ __forceinline__ GpProp prop_porosity(const double *vp, double fluid_sw) {
+  const double div43 = 0.133333333333333e+01, div23 = 0.666666666666667;
+  GpProp g;
+  const double modK = vp[3], mu = vp[4];
+  g.ibiot = vp[5];
+  g.cbiot = vp[6];
+  g.a = modK + div43 * mu;
+  g.b = (modK - div23 * mu) / g.a;
+  g.c = mu / g.a;
+  g.perm = vp[1] / fluid_sw;
+  return g;
+}
+
+// konzey_carman (Poromec.f:18-30)
+__device__ __forceinline__ double konzey_carman(double perm0, double poro, double poro0) {
+  const double tmp1 = (poro * poro * poro) / ((1.0 - poro) * (1.0 - poro));
+  const double tmp2 = ((1.0 - poro0) * (1.0 - poro0)) / (poro0 * poro0 * poro0);
+  return perm0 * tmp1 * tmp2;
+}
+
+// hashin_shtrikman (Poromec.f:113-163): drained K, mu, 1/M, alpha of a solid
+// of moduli ks, mus at porosity poro
+__device__ __forceinline__ void hashin_shtrikman(double poro, double ks, double mus, double &k,
+                                                 double &mu, double &im_biot, double &c_biot) {
+  const double ha = 1.0 - poro;
+  const double hb = 9.0 + 6.0 * poro;
+  const double hc = 8.0 + 12.0 * poro;
+  k = (4.0 * ks * mus * ha) / (3.0 * ks * poro + 4.0 * mus);
+  mu = (mus * ha * (9.0 * ks + 8 * mus)) / (ks * hb + mus * hc);
+  c_biot = 1.0 - k / ks;
+  im_biot = (c_biot - poro) / ks;
+}
+
+// extrapol_hexa20_v2 (Elmlib.f:3540-3590): the trilinear fit through the
+// values f[0..7] at the corner integration points 1, 4, 13, 16, 49, 52, 61,
+// 64, evaluated at vertex v
+__device__ __forceinline__ double extrapol_vertex(const double *f, int v) {
+  const double c1 = 0.125000000000000, c2 = 0.145157042290566;
+  const double c3 = 0.168564535400043, c4 = 0.195746635145486;
+  const double a1 = c1 * (f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7]);
+  const double a2 = c2 * (f[1] + f[3] + f[5] + f[7] - (f[0] + f[2] + f[4] + f[6]));
+  const double a3 = c2 * (f[2] + f[3] + f[6] + f[7] - (f[0] + f[1] + f[4] + f[5]));
+  const double a4 = c2 * (f[4] + f[5] + f[6] + f[7] - (f[0] + f[1] + f[2] + f[3]));
+  const double a5 = c3 * (f[0] + f[3] + f[4] + f[7] - (f[1] + f[2] + f[5] + f[6]));
+  const double a6 = c3 * (f[0] + f[1] + f[6] + f[7] - (f[2] + f[3] + f[4] + f[5]));
+  const double a7 = c3 * (f[0] + f[2] + f[5] + f[7] - (f[1] + f[3] + f[4] + f[6]));
+  const double a8 = c4 * (f[1] + f[2] + f[4] + f[7] - (f[0] + f[3] + f[5] + f[6]));
+  const double rnj = c_rn[v], snj = c_sn[v], tnj = c_tn[v];
+  return a1 + a2 * rnj + a3 * snj + a4 * tnj + a5 * rnj * snj + a6 * snj * tnj +
+         a7 * rnj * tnj + a8 * rnj * snj * tnj;
+}
+// the corner integration points (0-based) of the 4x4x4 rule, lx fastest
+static __constant__ int c_gp_corner[8] = {0, 3, 12, 15, 48, 51, 60, 63};

@@ -146,6 +146,14 @@ struct pm_ctx {
   int64_t *d_opptr = nullptr, *d_opsrc = nullptr;
   double *d_ke = nullptr, *d_pe = nullptr, *d_pv = nullptr;  // element arrays
   bool have_ordered_maps = false;
+  // variable properties (elmt18_pm): the conseq flags, element type per
+  // material, the 7 properties of every element's 64 integration points,
+  // and the materials as pm_set_materials received them (initial_prop
+  // rewrites the device copy)
+  int vprop[4] = {0, 0, 0, 0};
+  int *d_ie = nullptr;
+  double *d_vp = nullptr;
+  std::vector<double> h_e;
   // stress recovery: (element, corner) pairs per node and the outputs
   int64_t *d_tf_ptr = nullptr;
   int *d_tf_src = nullptr;
@@ -212,6 +220,7 @@ int pm_solve_dev(pm_ctx *c, const pm_solver_cfg *cfg, const double *b, double *x
 int pm_pform_dev(pm_ctx *c, int isw, int lhs, int rhs, double dt, double t, double *bvec);
 int pm_porosity_dev(pm_ctx *c, double dt);
 int pm_tform_dev(pm_ctx *c, double *tx, double *txe, double *txb, double *flux);
+int pm_update_prop_dev(pm_ctx *c, double dt);
 int pm_extract_diag(pm_ctx *c);
 int pm_make_val32(pm_ctx *c);
 int pm_step_dev(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_info *info);

@@ -356,7 +356,9 @@ int pm_step_dev(pm_ctx *c, double dt, const pm_solver_cfg *cfg, pm_step_info *in
   }
   if (rc == PM_OK) {
     hipEventRecord(ea, c->stream);
-    rc = pm_porosity_dev(c, dt);
+    // update_prop (isw 10) before the nodal porosity (Main.f:1070-1093)
+    rc = pm_update_prop_dev(c, dt);
+    if (rc == PM_OK) rc = pm_porosity_dev(c, dt);
     hipEventRecord(eb, c->stream);
     ms_elem += elapsed(c, ea, eb);
   }

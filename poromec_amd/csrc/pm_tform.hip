@@ -1,7 +1,8 @@
 // pm_tform.hip -- vertex stresses and Darcy flux of the current state, the
 // quantities the pnsf macro prints: tform_pm (Pform.f:433-700) with
 // elmt17_pm isw = 3 (Elmlib03.f:2206-2256, darcy_flux Elmlib.f:3178-3200)
-// and effective_stress (Pform.f:2487-2503).
+// or elmt18_pm isw = 3 (Elmlib03.f:3068-3160) and effective_stress
+// (Pform.f:2487-2503).
 //
 // The reference evaluates every element at its 8 vertices and adds the
 // values into the nodes, element by element, then divides by the number of
@@ -27,8 +28,85 @@ struct TformArgs {
   double gmod, grav[3];
   double *tx, *txe, *txb, *flux;  // (nnode, 6), (nnode, 6), (nnode, 6), (nnode, 3)
   int *err;
+  const int *ie;       // element type per material (elmt18_pm: 18)
+  const double *vp;    // (numel, 64, 7) integration-point properties
 };
 
+// elmt18_pm isw 3 (Elmlib03.f:3068-3160) at vertex v of element el: total
+// and Biot stress and Darcy flux at the 8 corner integration points (with
+// their properties), extrapolated to the vertex (extrapol_hexa20_v2)
+__device__ void corner_elmt18(const TformArgs &A, int el, int v, const int *nodes,
+                              const double (*xv)[3], const double *e, double *st, double *sb,
+                              double *fl) {
+  const double fluid_d = e[6] * 1.0e-06;
+  const double fluid_sw = fluid_d * A.gmod;
+  double F[15][8];
+  for (int j = 0; j < 8; j++) {
+    const int q = c_gp_corner[j];
+    const double r = c_pg4[q & 3], s = c_pg4[(q >> 2) & 3], t = c_pg4[q >> 4];
+    const GpProp G = prop_porosity(A.vp + ((size_t)el * 64 + q) * 7, fluid_sw);
+    double xji[3][3];
+    const double det = jacobian(xv, r, s, t, xji);
+    if (det <= 1.0e-14) atomicExch(A.err, PM_ERR_JACOBIAN);
+    double ux = 0, uy = 0, uz = 0, vx = 0, vy = 0, vz = 0, wx = 0, wy = 0, wz = 0;
+    for (int k = 0; k < 20; k++) {
+      double h, dr, ds, dt, gx, gy, gz;
+      dn20(k, r, s, t, h, dr, ds, dt);
+      to_global(xji, dr, ds, dt, gx, gy, gz);
+      const double *uk = A.u + (size_t)nodes[k] * 4;
+      const double u0 = uk[0], u1 = uk[1], u2 = uk[2];
+      ux = ux + gx * u0;
+      uy = uy + gy * u0;
+      uz = uz + gz * u0;
+      vx = vx + gx * u1;
+      vy = vy + gy * u1;
+      vz = vz + gz * u1;
+      wx = wx + gx * u2;
+      wy = wy + gy * u2;
+      wz = wz + gz * u2;
+    }
+    const double e0 = ux, e1 = vy, e2 = wz, e3 = uy + vx, e4 = vz + wy, e5 = uz + wx;
+    double t6[6];
+    t6[0] = (e0 + G.b * e1 + G.b * e2) * G.a;
+    t6[1] = (G.b * e0 + e1 + G.b * e2) * G.a;
+    t6[2] = (G.b * e0 + G.b * e1 + e2) * G.a;
+    t6[3] = e3 * G.c * G.a;
+    t6[4] = e4 * G.c * G.a;
+    t6[5] = e5 * G.c * G.a;
+    double dpm = 0.0, pi = 0.0;
+    for (int k = 0; k < 8; k++) dpm = dpm + n8(k, r, s, t) * A.dp[nodes[k]];
+    for (int k = 0; k < 8; k++) pi = pi + n8(k, r, s, t) * A.u[(size_t)nodes[k] * 4 + 3];
+    dpm = G.cbiot * dpm;
+    const double pm = G.cbiot * pi;
+    t6[0] = t6[0] - dpm;
+    t6[1] = t6[1] - dpm;
+    t6[2] = t6[2] - dpm;
+    for (int c = 0; c < 6; c++) {
+      F[c][j] = t6[c];
+      F[6 + c][j] = c < 3 ? t6[c] + pm : t6[c];
+    }
+    double fx = fluid_d * A.grav[0], fy = fluid_d * A.grav[1], fz = fluid_d * A.grav[2];
+    for (int k = 0; k < 8; k++) {
+      double dr, ds, dt, gx, gy, gz;
+      dn8(k, r, s, t, dr, ds, dt);
+      to_global(xji, dr, ds, dt, gx, gy, gz);
+      const double pk = A.u[(size_t)nodes[k] * 4 + 3];
+      fx = fx - gx * pk;
+      fy = fy - gy * pk;
+      fz = fz - gz * pk;
+    }
+    F[12][j] = G.perm * fx;
+    F[13][j] = G.perm * fy;
+    F[14][j] = G.perm * fz;
+  }
+  for (int c = 0; c < 6; c++) {
+    st[c] = extrapol_vertex(F[c], v);
+    sb[c] = extrapol_vertex(F[6 + c], v);
+  }
+  for (int c = 0; c < 3; c++) fl[c] = extrapol_vertex(F[12 + c], v);
+}
+
+template <bool V18>
 __global__ __launch_bounds__(256) void k_tform(TformArgs A) {
   // each XCD takes a contiguous range of nodes, so the elements that
   // neighbouring nodes share are gathered through one L2
@@ -44,6 +122,18 @@ __global__ __launch_bounds__(256) void k_tform(TformArgs A) {
     for (int k = 0; k < 8; k++)
       for (int d = 0; d < 3; d++) xv[k][d] = A.xv[(size_t)nodes[k] * 3 + d];
     const double *e = A.e + (size_t)A.mat[el] * PM_PROP;
+    if (V18 && A.ie[A.mat[el]] == 18) {
+      double st[6], sb[6], fl[3];
+      corner_elmt18(A, el, v, nodes, xv, e, st, sb, fl);
+      for (int j = 0; j < 6; j++) {
+        t[j] = t[j] + st[j];
+        tb[j] = tb[j] + sb[j];
+      }
+      f[0] = f[0] + fl[0];
+      f[1] = f[1] + fl[1];
+      f[2] = f[2] + fl[2];
+      continue;
+    }
     // material constants of elmt17_pm (Elmlib03.f:1962-2000)
     const double fluid_d = e[6] * 1.0e-06;
     const double fluid_sw = fluid_d * A.gmod;
@@ -185,6 +275,9 @@ int pm_tform_dev(pm_ctx *c, double *tx, double *txe, double *txb, double *flux) 
   A.txb = A.txe + (size_t)nn * 6;
   A.flux = A.txb + (size_t)nn * 6;
   A.err = c->d_elem_err;
+  A.ie = c->d_ie;
+  A.vp = c->d_vp;
+  const bool v18 = c->vprop[0] != 0;
   PM_CHECK_HIP(hipMemsetAsync(c->d_elem_err, 0, sizeof(int), c->stream));
   // the vertices (nodes 1..nnodev; promotion numbers the mid-edge nodes
   // after them) carry all the element work: their launch is spread over the
@@ -194,11 +287,12 @@ int pm_tform_dev(pm_ctx *c, double *tx, double *txe, double *txb, double *flux) 
   A.n0 = 0;
   A.n1 = nv;
   const int gv = (((nv + 255) / 256) + 7) & ~7;
-  hipLaunchKernelGGL(k_tform, dim3(gv), dim3(256), 0, c->stream, A);
+  if (v18) hipLaunchKernelGGL(k_tform<true>, dim3(gv), dim3(256), 0, c->stream, A);
+  else hipLaunchKernelGGL(k_tform<false>, dim3(gv), dim3(256), 0, c->stream, A);
   if (nn > nv) {
     A.n0 = nv;
     A.n1 = nn;
-    hipLaunchKernelGGL(k_tform, dim3((nn - nv + 255) / 256), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(k_tform<false>, dim3((nn - nv + 255) / 256), dim3(256), 0, c->stream, A);
   }
   PM_CHECK_HIP(hipGetLastError());
   int err = 0;

@@ -51,13 +51,14 @@ import numpy as np
 from .problem import NDF, NEN, NPAR, NTERMS, NUMLOAD, PROP, Deck, empty_load_table
 from .solver import SolverConfig
 
-# element type ids accepted for the hexa20/8 Biot element (elmt17_pm)
-ELEMENT_TYPES = (17, 7)
+# element type ids accepted: the hexa20/8 Biot element (elmt17_pm; 7 in the
+# shipped decks) and its variable-property form (elmt18_pm)
+ELEMENT_TYPES = (17, 7, 18)
 # post-processing macros: they write files this path does not produce
 OUTPUT_MACROS = ("pgeo", "pres", "setprint", "setpi", "ppi", "pcoo", "pcolors", "pmecres",
                  "ptermres", "deltatc")
 # macros that change the problem beyond this path
-UNSUPPORTED_TOP = ("block_pu", "conseq", "hextotet", "solvt", "solvm", "presolv", "neqs", "run")
+UNSUPPORTED_TOP = ("block_pu", "hextotet", "solvt", "solvm", "presolv", "neqs", "run")
 UNSUPPORTED_MESH = ("hexa20", "bar2", "tria3", "quad4", "tetra4", "tetra10", "constraindisp",
                     "nodalforces", "nodalloads", "constraintemp", "nodalthermloads",
                     "hydrostress", "initialstress", "parallel", "tria3ov", "quad4ov",
@@ -159,6 +160,8 @@ class MefInput:
     skipped: list = field(default_factory=list)
     element_type: int = 17
     path: str = ""
+    ie: np.ndarray = None          # (numat) element type per material, 17 or 18
+    vprop: tuple = (False, False, False, False)  # conseq: vprop, kozeny_carman, density, hashin_shtrikman
 
 
 def read_deck(path) -> MefInput:
@@ -167,7 +170,7 @@ def read_deck(path) -> MefInput:
     cfg = SolverConfig(solver="sqmr", precond=2, tol=1.0e-11, maxit=50000, ngram=150,
                        maxnlit=500, nltol=1.0e-4, stop_crit=2)  # Main.f:262-289 defaults
     st = dict(dt=1.0, gravity=np.array([0.0, 0.0, -9.81]), reorder=True, mesh=None,
-              pnodes=[], skipped=[])
+              pnodes=[], skipped=[], vprop=[False] * 4)
     st["gravity_mod"] = float(np.linalg.norm(st["gravity"]))
     program = []
     loop = None  # (count, [ops]) while inside loop ... next
@@ -200,6 +203,10 @@ def read_deck(path) -> MefInput:
             _read_config(src, w, st)
         elif mc == "gravity":
             _read_gravity(src, w, st)
+        elif mc == "conseq":
+            if st["mesh"] is not None:
+                raise DeckError("This macro can only be used before macro mesh")
+            st["vprop"] = _read_conseq(src.path(w[1]) if len(w) > 1 else "")
         elif mc == "solver":
             if st["mesh"] is not None:
                 raise DeckError("This macro can only be used before macro mesh")
@@ -238,9 +245,14 @@ def read_deck(path) -> MefInput:
         deck.dt = dts[0]
     if cfg.solver == "pcg" and cfg.precond not in (1, 2, 5):
         raise DeckError(f"preconditioner {cfg.precond} is outside this path")
+    ie = np.where(m["ie"] == 7, 17, m["ie"]).astype(np.int32)
+    any18 = bool((ie == 18).any())
+    if st["vprop"][0] != any18:  # check_element, Rdata.f:3744-3758
+        raise DeckError("Erro: Invalid elements!! (elmt18 materials go with conseq vprop, "
+                        "elmt17 without it)")
     return MefInput(deck=deck, u0=m["u0"], cfg=cfg, reorder=st["reorder"], pnodes=st["pnodes"],
                     program=program, skipped=st["skipped"], element_type=m["etype"],
-                    path=str(path))
+                    path=str(path), ie=ie, vprop=tuple(st["vprop"]))
 
 
 # -- top-level macros ----------------------------------------------------------------
@@ -289,6 +301,28 @@ def _read_gravity(src, w, st):
     else:
         raise DeckError("Erro na leitura da macro (GRAVITY)")
     st["gravity"], st["gravity_mod"] = g, mg
+
+
+def _read_conseq(path):
+    """``conseq <file>``: the constitutive options, one word a line until
+    ``end`` (read_constitutive_equation, Rdata.f:2545-2640); any of
+    kozeny_carman, density, hashin_shtrikman turns vprop on."""
+    try:
+        words = _file_words(path)
+    except OSError:
+        raise DeckError(f"File {path} not found !") from None
+    flags = [False] * 4
+    names = ("vprop", "kozeny_carman", "density", "hashin_shtrikman")
+    for wd in words:
+        if wd == "end":
+            break
+        if wd == "plastic":
+            raise DeckError("plasticity (conseq plastic) is outside the mef Biot u-p path")
+        if wd in names:
+            flags[names.index(wd)] = True
+    if any(flags[1:]):
+        flags[0] = True
+    return flags
 
 
 def _read_solver_file(path, cfg):
@@ -491,7 +525,8 @@ def _fmaterials(src, m, numat):
 
 def _element_type(m, ty):
     if ty not in ELEMENT_TYPES:
-        raise DeckError(f"element type {ty}: this path is the hexa20/8 Biot element (17)")
+        raise DeckError(f"element type {ty}: this path is the hexa20/8 Biot element "
+                        "(17, or 18 with properties at the integration points)")
     m["etype"] = ty
 
 
@@ -690,12 +725,15 @@ def truncate(inp: MefInput, nsteps: int) -> MefInput:
 
 
 def write_deck(path, deck: Deck, cfg: SolverConfig | None = None, u0=None, reorder=True,
-               pnodes=(1,), nsteps=None, element_type=17):
+               pnodes=(1,), nsteps=None, element_type=17, vprop=None):
     """Writes ``deck`` as an input deck of the reference in its current
     grammar: the main file ``path`` and the files it names beside it
     (``<stem>_config.dat``, ``_solver.dat``, ``_gravity.dat``, ``_node.dat``,
-    ``_load<nc>.dat``).  The program is ``dt``, ``pnup``, then ``nsteps``
-    (default ``deck.nsteps``) times ``solv`` + ``pnup``.  Returns the path."""
+    ``_load<nc>.dat``, ``_conseq.dat``).  ``element_type`` is one type for
+    every material or a list per material; ``vprop`` the conseq flags
+    (vprop, kozeny_carman, density, hashin_shtrikman).  The program is
+    ``dt``, ``pnup``, then ``nsteps`` (default ``deck.nsteps``) times
+    ``solv`` + ``pnup``.  Returns the path."""
     cfg = cfg or SolverConfig()
     nsteps = deck.nsteps if nsteps is None else nsteps
     path = os.fspath(path)
@@ -718,13 +756,18 @@ def write_deck(path, deck: Deck, cfg: SolverConfig | None = None, u0=None, reord
     L.append("gravity " + side("gravity", f"{_f(g[0])} {_f(g[1])} {_f(g[2])} "
                                           f"{_f(deck.gravity_mod)}\n"))
     L.append(f"nl {cfg.maxnlit} {_f(cfg.nltol)} {cfg.stop_crit}")
+    if vprop is not None and any(vprop):
+        names = ("vprop", "kozeny_carman", "density", "hashin_shtrikman")
+        L.append("conseq " + side("conseq", "".join(f"{n}\n" for n, f in zip(names, vprop) if f)
+                                  + "end\n"))
     L.append("mesh")
     L.append(f"nnode {nnode} numel {numel} numat {numat} maxno {NEN} ndf {NDF} dim 3")
     L.append("materials")
+    types = np.broadcast_to(np.asarray(element_type), (numat,))
     for ma in range(numat):
         row = deck.e[ma]
         last = max(7, int(np.nonzero(row)[0].max()) + 1 if row.any() else 7)
-        L.append(f"{ma + 1} {element_type} " + " ".join(_f(v) for v in row[:last]))
+        L.append(f"{ma + 1} {int(types[ma])} " + " ".join(_f(v) for v in row[:last]))
     L.append("end materials")
     L.append("coordinates")
     L.extend(f"{k + 1} {_f(x[0])} {_f(x[1])} {_f(x[2])}" for k, x in enumerate(deck.x))

@@ -94,6 +94,8 @@ def run(inp: MefInput, prefix: str, precision: str = "fp64", ordered: bool = Fal
     try:
         if ordered:
             s.set_ordered(True)
+        if inp.vprop[0]:
+            s.set_vprop(inp.ie, inp.vprop)
         nv = deck.nnodev
         u0 = np.zeros((s.nnode, NDF))
         u0[:nv] = inp.u0

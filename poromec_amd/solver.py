@@ -96,6 +96,20 @@ class MefSolver:
                                           L.ptr(L.as_f64(deck.gravity)), float(deck.gravity_mod)),
                 "pm_set_materials")
 
+    def set_vprop(self, ie, vprop):
+        """Element type per material (17 / 18) and the conseq flags
+        (vprop, kozeny_carman, density, hashin_shtrikman): properties at the
+        integration points (elmt18_pm), set up as initial_prop does."""
+        self.ie = L.as_i32(ie)
+        fl = L.as_i32([int(bool(v)) for v in vprop])
+        L.check(self.lib.pm_set_vprop(self.ctx, L.ptr(self.ie), L.ptr(fl)), "pm_set_vprop")
+
+    def vprop(self):
+        """The integration points' properties, (numel, 64, 7)."""
+        v = np.zeros((self.deck.numel, 64, 7))
+        L.check(self.lib.pm_get_vprop(self.ctx, L.ptr(v)), "pm_get_vprop")
+        return v
+
     def numbering(self):
         ix20 = np.zeros((self.deck.numel, NEN + 1), np.int32)
         idm = np.zeros((self.nnode, NDF), np.int32)

@@ -49,9 +49,11 @@ def oracle_run(inp, nsteps=None, stress=None):
     c = inp.cfg
     deck = inp.deck
     st = O.build_structure(deck, reorder=inp.reorder)
+    vp = inp.vprop if inp.vprop[0] else None
     m = O.OracleModel(deck, st=st, solver=SOLVERS[c.solver], precond=c.precond, solvtol=c.tol,
                       maxit=c.maxit, ngram=c.ngram, maxnlit=c.maxnlit, tol=c.nltol,
-                      stop_crit=c.stop_crit, maxnlit_accept=c.maxnlit_accept)
+                      stop_crit=c.stop_crit, maxnlit_accept=c.maxnlit_accept,
+                      ie=inp.ie if vp else None, vprop=vp)
     nv = deck.nnodev
     m.u0[:nv] = inp.u0
     m.u[:nv] = inp.u0

@@ -273,3 +273,34 @@ def test_read_printnode_round_trip(tmp_path):
     st, t, u = read_printnode(p)
     assert st.tolist() == [0, 1] and t.tolist() == [0.0, 3600.0]
     assert np.allclose(u, [r[2] for r in rows], rtol=1e-10, atol=0)
+
+
+def test_conseq_and_elmt18_round_trip(tmp_path):
+    d = P.column_deck(2, 2, 3)
+    d.e[:, 7] = 0.3
+    p = write_deck(tmp_path / "v.dat", d, SolverConfig(stop_crit=1), element_type=18,
+                   vprop=(True, True, False, True), nsteps=2)
+    inp = read_deck(p)
+    assert inp.ie.tolist() == [18] * d.e.shape[0]
+    assert inp.vprop == (True, True, False, True)
+    assert np.array_equal(inp.deck.e, d.e)
+
+
+@pytest.mark.parametrize("etype,flags,msg", [
+    (18, None, "Invalid elements"),
+    (17, (True, False, False, False), "Invalid elements"),
+])
+def test_conseq_element_check(tmp_path, etype, flags, msg):
+    d = P.column_deck(2, 2, 3)
+    p = write_deck(tmp_path / "v.dat", d, element_type=etype, vprop=flags, nsteps=1)
+    with pytest.raises(DeckError, match=msg):
+        read_deck(p)
+
+
+def test_conseq_plastic_rejected(tmp_path):
+    d = P.column_deck(2, 2, 3)
+    p = write_deck(tmp_path / "v.dat", d, element_type=18, vprop=(True, False, False, False),
+                   nsteps=1)
+    (tmp_path / "v_conseq.dat").write_text("plastic\nvprop\nend\n")
+    with pytest.raises(DeckError, match="plastic"):
+        read_deck(p)
