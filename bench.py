@@ -308,6 +308,49 @@ def tform_form(args):
     s.close()
 
 
+def vprop_form(args):
+    """The variable-property element (elmt18_pm, every conseq flag on)
+    against elmt17_pm on the n^3 box: one FP64 step each from the same
+    state, with the step's element and solver time; the oracle's step with
+    and without the properties timed on a small box beside it."""
+    from oracle import oracle as O
+    from poromec_amd import problem as P
+    from poromec_amd.solver import MefSolver, SolverConfig
+
+    def deck(n):
+        d = P.heterogeneous_box_deck(n, seed=args.seed, dt=0.1, patch_fraction=0.1, nsteps=1)
+        d.e[:, 5], d.e[:, 7] = 2000.0, 0.3
+        return d
+
+    n = args.n
+    cfg = SolverConfig(tol=1e-10, stop_crit=1)
+    out = {"form": "vprop", "n": n}
+    for label, ie in (("elmt17", None), ("elmt18", 18)):
+        d = deck(n)
+        s = MefSolver(d)
+        if ie:
+            s.set_vprop([18] * d.e.shape[0], (1, 1, 1, 1))
+        info = s.step(cfg)
+        out[label] = {"ms_total": info["ms_total"], "ms_element": info["ms_element"],
+                      "ms_solve": info["ms_solve"], "nlit": info["nlit"], "iters": info["iters"]}
+        log(f"{label} {n}^3: step {info['ms_total']:.1f} ms, elements {info['ms_element']:.1f} ms, "
+            f"solves {info['ms_solve']:.1f} ms, iterations {info['iters']}")
+        out["numel"] = d.numel
+        s.close()
+    m_small = max(3, min(n, 6))
+    cpu = {}
+    for label, kw in (("elmt17", {}), ("elmt18", {"ie": [18], "vprop": (1, 1, 1, 1)})):
+        dsm = deck(m_small)
+        dsm.e = dsm.e[:1]
+        dsm.ix[:, 20] = 1
+        m = O.OracleModel(dsm, stop_crit=1, **kw)
+        t0 = time.perf_counter()
+        m.step()
+        cpu[label] = time.perf_counter() - t0
+    out["cpu_oracle"] = {"box": m_small, "s_per_step": cpu, "cores": 1, "kind": "port"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,7 +361,7 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--cpu-sample-n", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", choices=["precision", "batch", "adjoint", "tform"],
+    ap.add_argument("--form", choices=["precision", "batch", "adjoint", "tform", "vprop"],
                     default="precision",
                     help="precision (default, the metric's workload), batch or adjoint form")
     ap.add_argument("--nbox", type=int, default=256)
@@ -346,6 +389,8 @@ def main():
         return adjoint_form(args)
     if args.form == "tform":
         return tform_form(args)
+    if args.form == "vprop":
+        return vprop_form(args)
 
     from poromec_amd import problem as P
     from poromec_amd.solver import MefSolver, SolverConfig
