@@ -136,6 +136,24 @@ def test_fp32ir_reaches_tolerance(column):
     assert np.abs(x32 - x64).max() <= 1e-8 * np.abs(x64).max()
 
 
+def test_fp32ir_every_spmv_shape():
+    """Every SpMV shape, the software-pipelined ones included, on a box whose
+    interior node blocks are wider than one pass of the narrower shapes
+    (270 columns): FP32+IR reaches the FP64 solution."""
+    from poromec_amd.solver import SPMV_SHAPES, SolverConfig
+    d = P.heterogeneous_box_deck(4, seed=2, sigma_lnk=0.5, corr_len=2.0)
+    s = _solver(d)
+    s.pform(2, True, False, d.dt, d.dt)
+    assert s.matrix_info()["ncols"] > 0
+    b = np.random.default_rng(6).standard_normal(s.neq)
+    x64, _ = s.solve(b, SolverConfig(tol=1e-10))
+    for shape in range(SPMV_SHAPES):
+        s.set_spmv_shape(True, shape)
+        x32, i32 = s.solve(b, SolverConfig(tol=1e-10, precision="fp32ir"))
+        assert i32["true_rel"] < 1e-10, shape
+        assert np.abs(x32 - x64).max() <= 1e-8 * np.abs(x64).max(), shape
+
+
 def test_csrc_dropin_solve(column):
     from poromec_amd.solver import CsrcSolver, SolverConfig
     d, st = column
