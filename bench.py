@@ -380,7 +380,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this path has no multi-GPU form "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+              "(it would shard by domain decomposition, one subdomain per GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.form == "batch":
