@@ -66,15 +66,17 @@ FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
 
 
 def build_fortran() -> Path | None:
-    """The Fortran host (iso_c_binding module + time-march driver), linked
-    against libpmhip.so; skipped when amdflang is absent."""
+    """The Fortran host (iso_c_binding module, deck reader and the
+    pm_mefdrive driver), linked against libpmhip.so; skipped when amdflang
+    is absent."""
     fdir = HERE / "fortran"
     exe = fdir / "pm_mefdrive"
     if not Path(FC).exists():
         return None
     mod_dir = fdir / "_build"
     mod_dir.mkdir(exist_ok=True)
-    cmd = [FC, "-O2", "-J", str(mod_dir), str(fdir / "pmhip.f90"), str(fdir / "pm_mefdrive.f90"),
+    cmd = [FC, "-O2", "-J", str(mod_dir), str(fdir / "pmhip.f90"), str(fdir / "pm_deck.f90"),
+           str(fdir / "pm_mefdrive.f90"),
            "-L", str(HERE), "-lpmhip", "-Wl,-rpath,$ORIGIN/..", "-o", str(exe)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -121,6 +121,28 @@ module pmhip
       type(pm_solver_cfg) :: cfg
       type(pm_step_info) :: info
     end function
+    integer(c_int) function pm_tform(ctx, tx, txe, txb, flux) bind(C, name='pm_tform')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: tx(*), txe(*), txb(*), flux(*)
+    end function
+    integer(c_int) function pm_set_vprop(ctx, ie, vprop) bind(C, name='pm_set_vprop')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: ie(*), vprop(4)
+    end function
+    integer(c_int) function pm_set_ordered(ctx, on) bind(C, name='pm_set_ordered')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function pm_matrix_info(ctx, neq, nblocks, nnz, ncols) &
+        bind(C, name='pm_matrix_info')
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int) :: neq, nblocks
+      integer(c_int64_t) :: nnz, ncols
+    end function
     type(c_ptr) function pm_error_string(code) bind(C, name='pm_error_string')
       import :: c_ptr, c_int
       integer(c_int), value :: code

@@ -1,0 +1,91 @@
+"""The Fortran host as the reference's ``mef <input> <output>``: pm_mefdrive
+reads a deck with its own reader (pm_deck.f90), drives the GPU through the
+pmhip iso_c_binding module and writes the output files with Fortran edit
+descriptors.  Both hosts call the same deterministic library in the same
+order, so every history and log file must equal, byte for byte, the one the
+Python runner (poromec_amd.mef) writes for the same deck -- whose histories
+the deck tests hold against the oracle and the reference's printed
+results."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from deckcases import extract, reference
+from poromec_amd import problem as P
+from poromec_amd.deck import read_deck, truncate, write_deck
+from poromec_amd.mef import PNSF_FILES, read_printnode, run
+from poromec_amd.solver import SolverConfig
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parents[1]
+EXE = ROOT / "poromec_amd" / "fortran" / "pm_mefdrive"
+
+
+def _run_both(path, tmp_path, nsteps, fp32ir=False):
+    if not EXE.exists():
+        pytest.skip("Fortran host not built (amdflang absent)")
+    inp = read_deck(path)
+    if nsteps:
+        inp = truncate(inp, nsteps)
+    py = str(tmp_path / "py")
+    run(inp, py, precision="fp32ir" if fp32ir else "fp64")
+    fo = str(tmp_path / "fo")
+    args = [str(EXE), str(path), fo, str(nsteps or 0)] + (["fp32ir"] if fp32ir else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    names = [f"_up_node_{n}.txt" for n in inp.pnodes]
+    if any(op[0] == "pnsf" for op in inp.program):
+        names += [f"_{stem}_node_{n}.txt" for stem, _ in PNSF_FILES for n in inp.pnodes]
+    names += ["_log_1.txt", "_log_nonlinear_1.txt"]
+    for name in names:
+        a, b = Path(py + name).read_text(), Path(fo + name).read_text()
+        assert a == b, (name, [(x, y) for x, y in zip(a.splitlines(), b.splitlines()) if x != y][:3])
+    tf = Path(fo + "_t_1.txt").read_text().splitlines()
+    tp = Path(py + "_t_1.txt").read_text().splitlines()
+    assert tf[0] == tp[0] and tf[-5:] == tp[-5:]  # the system sizes
+    return inp, fo
+
+
+@pytest.mark.parametrize("case,nsteps", [("exemplo1_mesh0_t2", 0), ("solo1", 6),
+                                         ("solo_cilindro_hexa_1", 6)])
+def test_fortran_host_deck(tmp_path, case, nsteps):
+    print(f"deck {case} steps {nsteps or 'all'}", flush=True)
+    path = extract(case, tmp_path)
+    inp, fo = _run_both(path, tmp_path, nsteps)
+    ref = reference(case)
+    # the reference's printed histories, as printed by this host
+    for n in inp.pnodes:
+        st, t, u = read_printnode(f"{fo}_up_node_{n}.txt")
+        r = ref["hist"][n][: len(st)]
+        assert np.array_equal(st, r[:, 0].astype(int))
+        assert np.allclose(u, r[:, 2:], rtol=5e-9, atol=1e-9 * np.abs(r[:, 2:]).max())
+
+
+def test_fortran_host_fp32ir(tmp_path):
+    """FP32 matrix values inside FP64 refinement (SQMR, tol 1e-10)."""
+    print("written column deck, fp32ir, 4 steps", flush=True)
+    deck = P.column_deck(2, 2, 6, dt=0.05, load_mpa=0.5)
+    path = write_deck(tmp_path / "c.dat", deck, SolverConfig(stop_crit=1), pnodes=(1, 7),
+                      nsteps=4)
+    _run_both(path, tmp_path, 0, fp32ir=True)
+
+
+def test_fortran_host_vprop_deck(tmp_path):
+    """elmt18 (properties at the integration points, Kozeny-Carman and
+    Hashin-Shtrikman) and a time-table load through the Fortran host."""
+    print("written elmt18 deck, 5 steps", flush=True)
+    deck = P.column_deck(2, 2, 6, dt=0.05, load_mpa=0.5)
+    deck.e[:, 5] = 2000.0
+    deck.e[:, 7] = 0.3
+    u0 = np.zeros((deck.nnodev, 4))
+    u0[:, 3] = 1.0e3
+    path = write_deck(tmp_path / "v18.dat", deck, SolverConfig(stop_crit=1), u0=u0,
+                      pnodes=(1, deck.nnodev), nsteps=5, element_type=18,
+                      vprop=(True, True, False, True))
+    with open(path) as f:
+        text = f.read()
+    Path(path).write_text(text.replace("pnup\nnext", "pnup\npnsf\nnext"))
+    inp, _ = _run_both(path, tmp_path, 0)
+    assert inp.vprop[0] and (inp.ie == 18).all()
