@@ -75,7 +75,7 @@ def build_fortran() -> Path | None:
         return None
     mod_dir = fdir / "_build"
     mod_dir.mkdir(exist_ok=True)
-    cmd = [FC, "-O2", "-J", str(mod_dir), str(fdir / "pmhip.f90"), str(fdir / "pm_deck.f90"),
+    cmd = [FC, "-O2", "-J", str(mod_dir), str(fdir / "pmhip.f90"), str(fdir / "pm_deck.f90"), str(fdir / "pm_vtk.f90"),
            str(fdir / "pm_mefdrive.f90"),
            "-L", str(HERE), "-lpmhip", "-Wl,-rpath,$ORIGIN/..", "-o", str(exe)]
     r = subprocess.run(cmd, capture_output=True, text=True)

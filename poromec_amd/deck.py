@@ -11,9 +11,10 @@ macros between them; the ``mesh`` macro reads the mesh block with
 settings and the list of time-step and print macros; ``poromec_amd.mef``
 runs that program on the GPU and writes the reference's output files.
 
-Macros read (top level): ``config``, ``solver``, ``gravity``, ``nl``,
-``dt``, ``mesh``, ``setpnode``, ``pnup``, ``pnsf``, ``solv``,
-``loop``/``next``, ``stop``.  Mesh block: the parameter line, ``materials``, ``fmaterials``,
+Macros read (top level): ``config`` (``reord``, ``bvtk``, ``legacy_vtk``),
+``solver``, ``gravity``, ``nl``, ``conseq``, ``dt``, ``mesh``, ``setpnode``,
+``setprint``, ``pnup``, ``pnsf``, ``pres``, ``solv``, ``loop``/``next``,
+``stop``.  Mesh block: the parameter line, ``materials``, ``fmaterials``,
 ``hexa8``, ``coordinates``, ``constrainpmec``, ``elmtloads``,
 ``elmtpresloads``, ``loads`` (types 1-8 inline, 39-43 from a file),
 ``initialpres``, ``hydrostatic``, ``insert``/``return``, ``end``.
@@ -34,7 +35,7 @@ also read, each where the current reader would reject or misread it:
   their ``end <name>`` line closes that block and not the mesh block.
 
 Macros that only write post-processing files this reader does not produce
-(``pgeo``, ``pres``, ``setprint``, ``deltatc``, ...) are recorded in
+(``pgeo``, ``deltatc``, ``setpi``, ...) are recorded in
 :attr:`MefInput.skipped` and otherwise ignored; macros that would change
 the system this path solves (initial stresses, nodal forces, hexa20 input,
 plasticity, ...) raise :class:`DeckError`.
@@ -50,13 +51,13 @@ import numpy as np
 
 from .problem import NDF, NEN, NPAR, NTERMS, NUMLOAD, PROP, Deck, empty_load_table
 from .solver import SolverConfig
+from .vtk import DEFAULT_FLAGS, PRINT_WORDS
 
 # element type ids accepted: the hexa20/8 Biot element (elmt17_pm; 7 in the
 # shipped decks) and its variable-property form (elmt18_pm)
 ELEMENT_TYPES = (17, 7, 18)
 # post-processing macros: they write files this path does not produce
-OUTPUT_MACROS = ("pgeo", "pres", "setprint", "setpi", "ppi", "pcoo", "pcolors", "pmecres",
-                 "ptermres", "deltatc")
+OUTPUT_MACROS = ("pgeo", "setpi", "ppi", "pcoo", "pcolors", "pmecres", "ptermres", "deltatc")
 # macros that change the problem beyond this path
 UNSUPPORTED_TOP = ("block_pu", "hextotet", "solvt", "solvm", "presolv", "neqs", "run")
 UNSUPPORTED_MESH = ("hexa20", "bar2", "tria3", "quad4", "tetra4", "tetra10", "constraindisp",
@@ -156,12 +157,15 @@ class MefInput:
     cfg: SolverConfig
     reorder: bool
     pnodes: list                   # nodes of setpnode (1-based)
-    program: list                  # ("solv",) / ("pnup",) / ("pnsf",) / ("dt", value)
+    program: list                  # ("solv",) / ("pnup",) / ("pnsf",) / ("pres",) / ("dt", value)
     skipped: list = field(default_factory=list)
     element_type: int = 17
     path: str = ""
     ie: np.ndarray = None          # (numat) element type per material, 17 or 18
     vprop: tuple = (False, False, False, False)  # conseq: vprop, kozeny_carman, density, hashin_shtrikman
+    print_flags: tuple = DEFAULT_FLAGS  # setprint (print_flag 1-13) as the pres macro uses it
+    vtk_legacy: bool = False       # config legacy_vtk: .vtk instead of .vtu
+    vtk_binary: bool = False       # config bvtk
 
 
 def read_deck(path) -> MefInput:
@@ -170,7 +174,8 @@ def read_deck(path) -> MefInput:
     cfg = SolverConfig(solver="sqmr", precond=2, tol=1.0e-11, maxit=50000, ngram=150,
                        maxnlit=500, nltol=1.0e-4, stop_crit=2)  # Main.f:262-289 defaults
     st = dict(dt=1.0, gravity=np.array([0.0, 0.0, -9.81]), reorder=True, mesh=None,
-              pnodes=[], skipped=[], vprop=[False] * 4)
+              pnodes=[], skipped=[], vprop=[False] * 4, print_flags=list(DEFAULT_FLAGS),
+              vtk_legacy=False, vtk_binary=False)
     st["gravity_mod"] = float(np.linalg.norm(st["gravity"]))
     program = []
     loop = None  # (count, [ops]) while inside loop ... next
@@ -191,7 +196,7 @@ def read_deck(path) -> MefInput:
                 continue
             program.extend(loop[1] * loop[0])
             loop = None
-        elif mc in ("solv", "pnup", "pnsf"):
+        elif mc in ("solv", "pnup", "pnsf", "pres"):
             ops.append((mc,))
         elif mc == "dt":
             if len(w) < 2:
@@ -225,8 +230,16 @@ def read_deck(path) -> MefInput:
             cfg.maxnlit, cfg.stop_crit, cfg.maxnlit_accept = inum(w[1]), 1, True
         elif mc == "setpnode":
             st["pnodes"] = _read_pnode(src.path(w[1]))
+        elif mc == "setprint":
+            if len(w) < 2:
+                raise DeckError("setprint without a file")
+            _read_setprint(src.path(w[1]), st)
         elif mc == "mesh":
             st["mesh"] = _read_mesh(src, st)
+            # rdat_pm fixes the printed nodes (print_quad); an elastic run
+            # prints no consolidation pressure nor plastic flag (Main.f:552-556)
+            st["print_flags"][9] = st["print_flags"][10] = False
+            st["quad"] = st["print_flags"][0]
         elif mc in OUTPUT_MACROS:
             st["skipped"].append(mc)
         elif mc in UNSUPPORTED_TOP:
@@ -250,9 +263,13 @@ def read_deck(path) -> MefInput:
     if st["vprop"][0] != any18:  # check_element, Rdata.f:3744-3758
         raise DeckError("Erro: Invalid elements!! (elmt18 materials go with conseq vprop, "
                         "elmt17 without it)")
+    if st["vtk_binary"] and not st["vtk_legacy"] and any(op[0] == "pres" for op in program):
+        raise DeckError("bvtk without legacy_vtk: the reference has no binary VTU writer")
     return MefInput(deck=deck, u0=m["u0"], cfg=cfg, reorder=st["reorder"], pnodes=st["pnodes"],
                     program=program, skipped=st["skipped"], element_type=m["etype"],
-                    path=str(path), ie=ie, vprop=tuple(st["vprop"]))
+                    path=str(path), ie=ie, vprop=tuple(st["vprop"]),
+                    print_flags=tuple(st["print_flags"]), vtk_legacy=st["vtk_legacy"],
+                    vtk_binary=st["vtk_binary"])
 
 
 # -- top-level macros ----------------------------------------------------------------
@@ -279,6 +296,10 @@ def _read_config(src, w, st):
                 raise DeckError(f"config: {key} without a value")
             if key == "reord":
                 st["reorder"] = flag(words[i + 1])
+            elif key == "bvtk":
+                st["vtk_binary"] = flag(words[i + 1])
+            elif key == "legacy_vtk":
+                st["vtk_legacy"] = flag(words[i + 1])
             elif key == "newton_raphson" and flag(words[i + 1]):
                 raise DeckError("newton_raphson (tangent rebuilt every iteration) is outside this path")
             i += 2
@@ -301,6 +322,33 @@ def _read_gravity(src, w, st):
     else:
         raise DeckError("Erro na leitura da macro (GRAVITY)")
     st["gravity"], st["gravity_mod"] = g, mg
+
+
+def _read_setprint(path, st):
+    """``setprint <file>``: the fields the pres macro writes, the first word
+    of each line until ``end`` (set_print_vtk_pm, Rdata.f:3383-3520); the
+    file clears every flag first."""
+    try:
+        with open(path, "r", errors="replace") as f:
+            lines = f.read().replace("\r", " ").split("\n")
+    except OSError:
+        raise DeckError(f"File {path} not found !") from None
+    flags = [False] * len(DEFAULT_FLAGS)
+    for ln in lines:
+        w = ln.split()
+        if not w:
+            continue
+        if w[0] == "end":
+            break
+        if w[0] in PRINT_WORDS:
+            flags[PRINT_WORDS.index(w[0])] = True
+    if st["mesh"] is not None:
+        if flags[0] != st["quad"]:
+            raise DeckError("setprint quadratic after the mesh macro (the printed nodes are "
+                            "fixed when the mesh is read)")
+        if flags[9] or flags[10]:
+            raise DeckError("pconsolidation / elplastic are fields of the plastic path")
+    st["print_flags"] = flags
 
 
 def _read_conseq(path):

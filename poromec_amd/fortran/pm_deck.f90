@@ -23,10 +23,14 @@ module pmdeck
   use iso_c_binding
   implicit none
   private
-  public :: deck_t, read_deck, OP_SOLV, OP_PNUP, OP_PNSF, OP_DT
+  public :: deck_t, read_deck, OP_SOLV, OP_PNUP, OP_PNSF, OP_DT, OP_PRES
 
   integer, parameter :: MAXW = 96, LLEN = 2048, MAXINC = 16
-  integer, parameter :: OP_SOLV = 1, OP_PNUP = 2, OP_PNSF = 3, OP_DT = 4
+  integer, parameter :: OP_SOLV = 1, OP_PNUP = 2, OP_PNSF = 3, OP_DT = 4, OP_PRES = 5
+  ! setprint words (set_print_vtk_pm), in print_flag order
+  character(len=14), parameter :: PRINT_WORDS(13) = [character(len=14) :: 'quadratic', &
+    'desloc', 'pressure', 'dpressure', 'totalstress', 'biotstress', 'terzaghistress', &
+    'darcyflux', 'porosity', 'pconsolidation', 'elplastic', 'temperature', 'heatFlux']
   real(c_double), parameter :: EPS = 2.0d0**(-53)
 
   type deck_t
@@ -42,6 +46,10 @@ module pmdeck
     integer(c_int) :: maxnlit = 500, stop_crit = 2, maxnlit_accept = 0
     real(c_double) :: tol = 1.0d-11, nltol = 1.0d-4
     integer, allocatable :: pnodes(:)
+    ! the pres macro: print_flag (Main.f:220-232 defaults), legacy_vtk, bvtk
+    logical :: fprint(13) = [.false., .true., .true., .false., .false., .false., .false., &
+                             .false., .false., .false., .false., .false., .false.]
+    logical :: vtk_legacy = .false., vtk_binary = .false.
     integer :: nprog = 0
     integer, allocatable :: op(:)
     real(c_double), allocatable :: opval(:)
@@ -285,6 +293,11 @@ contains
         call add_op(d, OP_PNUP, 0.0d0)
       case ('pnsf')
         call add_op(d, OP_PNSF, 0.0d0)
+      case ('pres')
+        call add_op(d, OP_PRES, 0.0d0)
+      case ('setprint')
+        if (nw < 2) call fail('setprint without a file')
+        call read_setprint(d, words(2), have_mesh)
       case ('dt')
         if (nw < 2) call fail('Error reading macro (DT) !')
         call add_op(d, OP_DT, fnum(words(2)))
@@ -339,6 +352,9 @@ contains
       case ('mesh')
         call read_mesh(d)
         have_mesh = .true.
+        ! an elastic run prints neither consolidation pressure nor plastic flag
+        d%fprint(10) = .false.
+        d%fprint(11) = .false.
       case ('block_pu', 'hextotet', 'solvt', 'solvm', 'presolv', 'neqs', 'run')
         call fail('macro ' // trim(mc) // ' is outside the mef Biot u-p path')
       case default
@@ -347,6 +363,8 @@ contains
     end do
     if (inloop) d%nprog = loop0   ! a loop without next adds nothing
     if (.not. allocated(d%pnodes)) allocate(d%pnodes(0))
+    if (d%vtk_binary .and. .not. d%vtk_legacy .and. any(d%op(1:d%nprog) == OP_PRES)) &
+      call fail('bvtk without legacy_vtk: the reference has no binary VTU writer')
     if (.not. allocated(d%op)) allocate(d%op(0), d%opval(0))
     do while (nstack > 0)
       close(units(nstack))
@@ -402,12 +420,20 @@ contains
         if (i + 1 > n) call fail('config: reord without a value')
         d%reorder = lflag(ws(i + 1))
         i = i + 2
+      case ('bvtk')
+        if (i + 1 > n) call fail('config: bvtk without a value')
+        d%vtk_binary = lflag(ws(i + 1))
+        i = i + 2
+      case ('legacy_vtk')
+        if (i + 1 > n) call fail('config: legacy_vtk without a value')
+        d%vtk_legacy = lflag(ws(i + 1))
+        i = i + 2
       case ('newton_raphson')
         if (i + 1 <= n) then
           if (lflag(ws(i + 1))) call fail('newton_raphson is outside this path')
         end if
         i = i + 2
-      case ('memory', 'omp_elmt', 'omp_solver', 'nth_elmt', 'nth_solver', 'bvtk', 'legacy_vtk')
+      case ('memory', 'omp_elmt', 'omp_solver', 'nth_elmt', 'nth_solver')
         i = i + 2
       case default
         i = i + 1
@@ -488,6 +514,44 @@ contains
       end select
       i = i + 2
     end do
+  end subroutine
+
+  ! setprint <file>: the first word of each line until end names a field the
+  ! pres macro writes; the file clears every flag first (set_print_vtk_pm,
+  ! Rdata.f:3383-3520).  After the mesh the printed nodes are fixed.
+  subroutine read_setprint(d, fname, have_mesh)
+    type(deck_t), intent(inout) :: d
+    character(len=*), intent(in) :: fname
+    logical, intent(in) :: have_mesh
+    logical :: fl(13)
+    integer :: u, ios, k, i, m
+    character(len=LLEN) :: l
+    character(len=256) :: w
+    open(newunit=u, file=trim(resolve(fname)), status='old', action='read', iostat=ios)
+    if (ios /= 0) call fail('File ' // trim(fname) // ' not found !')
+    fl = .false.
+    do
+      read(u, '(a)', iostat=ios) l
+      if (ios /= 0) exit
+      do i = 1, len(l)
+        if (l(i:i) == char(13) .or. l(i:i) == char(9)) l(i:i) = ' '
+      end do
+      l = adjustl(l)
+      m = index(l, ' ')
+      if (m <= 1) cycle
+      w = l(1:m - 1)
+      if (trim(w) == 'end') exit
+      do k = 1, 13
+        if (trim(w) == trim(PRINT_WORDS(k))) fl(k) = .true.
+      end do
+    end do
+    close(u)
+    if (have_mesh) then
+      if (fl(1) .neqv. d%fprint(1)) &
+        call fail('setprint quadratic after the mesh macro (the printed nodes are fixed)')
+      if (fl(10) .or. fl(11)) call fail('pconsolidation / elplastic are fields of the plastic path')
+    end if
+    d%fprint = fl
   end subroutine
 
   ! conseq <file> (read_constitutive_equation, Rdata.f:2545-2640)

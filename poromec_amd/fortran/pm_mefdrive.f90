@@ -1,17 +1,18 @@
 ! pm_mefdrive.f90 -- a Fortran host that runs the mef Biot u-p path on the
 ! MI355X through the pmhip module.
 !
-!   pm_mefdrive <deck> <prefix> [maxsteps] [fp32ir] [parse]
+!   pm_mefdrive <deck> <prefix> [maxsteps] [fp32ir] [parse | vtkcheck]
 !
 ! is the reference's `mef <input> <output>` (Main.f:336-402) for the macros
 ! module pmdeck reads: the deck's mesh block sets up the structure, the
 ! materials and the loads; its macro program then runs in order -- dt, solv
 ! (one backward-Euler step, Main.f:876-1123), pnup and pnsf (the printnode
-! histories of the setpnode nodes, Main.f:2310-2481).  maxsteps > 0 ends the
-! program before its (maxsteps+1)-th solv; fp32ir solves with FP32 matrix
-! values inside FP64 iterative refinement; parse only reads the deck and
-! writes what it read to <prefix>_deck.bin (no device calls).  Files written,
-! named as the
+! histories of the setpnode nodes, Main.f:2310-2481) and pres (the results
+! file, Main.f:1322-1545).  maxsteps > 0 ends the program before its
+! (maxsteps+1)-th solv; fp32ir solves with FP32 matrix values inside FP64
+! iterative refinement.  Two modes make no device calls: parse writes what
+! the reader read to <prefix>_deck.bin; vtkcheck writes results files of
+! synthetic fields on the deck's mesh.  Files written, named as the
 ! reference's Filenames.f names them:
 !   <prefix>_up_node_<n>.txt          u v w p per pnup, (i9,10e18.10)
 !   <prefix>_stress_node_<n>.txt      total stress per pnsf
@@ -19,6 +20,7 @@
 !   <prefix>_stressB_node_<n>.txt     Biot effective stress
 !   <prefix>_flux_node_<n>.txt        Darcy flux
 !   <prefix>_poro_node_<n>.txt        porosity
+!   <prefix>_istep_<n>.vtu            per pres (.vtk with legacy_vtk)
 !   <prefix>_log_1.txt                one line per linear solve
 !   <prefix>_log_nonlinear_1.txt      step, time and nonlinear iterations
 !   <prefix>_t_1.txt                  device times and system sizes
@@ -41,10 +43,11 @@ program pm_mefdrive
   use iso_c_binding
   use pmhip
   use pmdeck
+  use pmvtk
   implicit none
   character(len=1024) :: a1, a2, a3
   integer :: n1, maxsteps, ios, i
-  logical :: fp32ir, parse
+  logical :: fp32ir, parse, vcheck
 
   call get_command_argument(1, a1, n1)
   call get_command_argument(2, a2)
@@ -62,6 +65,7 @@ program pm_mefdrive
   maxsteps = 0
   fp32ir = .false.
   parse = .false.
+  vcheck = .false.
   do i = 3, command_argument_count()
     call get_command_argument(i, a3)
     select case (trim(a3))
@@ -69,6 +73,8 @@ program pm_mefdrive
       fp32ir = .true.
     case ('parse')
       parse = .true.
+    case ('vtkcheck')
+      vcheck = .true.
     case default
       read(a3, *, iostat=ios) maxsteps
       if (ios /= 0) then
@@ -79,6 +85,8 @@ program pm_mefdrive
   end do
   if (parse) then
     call dump_deck(a1, a2)
+  else if (vcheck) then
+    call vtk_check(a1, a2)
   else
     call run_deck(a1, a2, maxsteps, fp32ir)
   end if
@@ -150,7 +158,7 @@ contains
   ! precond, maxit, ngram, maxnlit, stop_crit, maxnlit_accept, npnodes,
   ! nprog (int32); tol, nltol, gravity(3), gmod (real64); pnodes, op (int32);
   ! opval (real64); ix, idv, ie, eload, eloadp, load (int32); x, e, u0,
-  ! fload (real64)
+  ! fload (real64); print_flag(13), legacy_vtk, bvtk (int32)
   subroutine dump_deck(path, prefix)
     character(len=*), intent(in) :: path, prefix
     type(deck_t) :: d
@@ -166,8 +174,51 @@ contains
     write(u) d%opval(1:d%nprog)
     write(u) d%ix, d%idv, d%ie, d%eload, d%eloadp, d%load
     write(u) d%x, d%e, d%u0, d%fload
+    write(u) merge(1_c_int, 0_c_int, d%fprint), merge(1_c_int, 0_c_int, d%vtk_legacy), &
+      merge(1_c_int, 0_c_int, d%vtk_binary)
     close(u)
     write(*, '(a,i10,a,i10,a,i8)') ' parsed: numel', d%numel, ' nnodev', d%nnodev, ' ops', d%nprog
+  end subroutine
+
+  ! the results writer on a deck's mesh with exactly representable synthetic
+  ! fields (no device calls): <prefix>_{a,b,x}{l,q}_istep_7.vtk|vtu for legacy
+  ! ASCII, legacy binary and VTU, over the vertices (l) and every node (q)
+  subroutine vtk_check(path, prefix)
+    character(len=*), intent(in) :: path, prefix
+    type(deck_t) :: d
+    integer(c_int), allocatable :: ix20(:), idn(:), perm(:)
+    real(c_double), allocatable :: f(:,:)
+    integer(c_int) :: nmax, nnode, neq, ierr
+    integer(c_int64_t) :: nad
+    integer :: i, k, iq, ie, code
+    logical :: fl(13), leg, bin
+    character(len=1) :: enc(3) = ['a', 'b', 'x'], qs(2) = ['l', 'q']
+    call read_deck(path, d)
+    nmax = d%nnodev + 12 * d%numel
+    allocate(ix20(21 * d%numel), idn(4 * nmax), perm(nmax))
+    ierr = pm_mesh_numbering(d%numel, d%nnodev, d%ix, d%idv, merge(1_c_int, 0_c_int, d%reorder), &
+                             nmax, ix20, idn, perm, nnode, neq, nad)
+    call check(ierr, 'pm_mesh_numbering')
+    allocate(f(27, nnode))
+    do i = 1, nnode
+      do k = 1, 27
+        f(k, i) = real(mod(i * 37 + k * 101, 2001) - 1000, c_double) * 2.0d0**(2 * k - 20)
+      end do
+    end do
+    do iq = 1, 2
+      fl = .true.
+      fl(1) = iq == 2
+      fl(10:13) = .false.
+      do ie = 1, 3
+        leg = ie /= 3
+        bin = ie == 2
+        call write_res(res_name(trim(prefix) // '_' // enc(ie) // qs(iq), 7, leg), ix20, d%x, &
+                       d%nnodev, f(1:4, :), f(5, :), f(6, :), f(7:12, :), f(13:18, :), &
+                       f(19:24, :), f(25:27, :), nnode, d%numel, 7, 12345.678d0, fl, leg, bin, code)
+        if (code /= 0) stop 3
+      end do
+    end do
+    write(*, '(a,i10)') ' vtk files written, nnode', nnode
   end subroutine
 
   subroutine run_deck(path, prefix, maxsteps, fp32ir)
@@ -181,6 +232,7 @@ contains
     integer(c_int) :: nnode, neq, nb, reorder, ierr
     integer(c_int64_t) :: nad, nnz, ncols
     real(c_double), allocatable :: u(:), u0(:), dp(:), poro(:), tx(:), txe(:), txb(:), flux(:)
+    integer(c_int), allocatable :: ix20(:), idn(:), perm(:)
     real(c_double) :: dt, t, tnow, ms_elem, ms_solve, ms_total
     integer :: iop, istep, k, flog, fnl, ftm, nsolv
     integer(8) :: c0, c1, crate
@@ -297,6 +349,32 @@ contains
         call print_nodes(prefix, 'flux', 'darcyFlux', first_sf, d%pnodes, istep, t, flux, 3, 3)
         call print_nodes(prefix, 'poro', 'poro', first_sf, d%pnodes, istep, t, poro, 1, 1)
         first_sf = .false.
+      case (OP_PRES)
+        if (.not. allocated(tx)) allocate(tx(6 * nnode), txe(6 * nnode), txb(6 * nnode), &
+                                          flux(3 * nnode))
+        if (any(d%fprint(5:8))) then
+          ierr = pm_tform(ctx, tx, txe, txb, flux)
+          call check(ierr, 'pm_tform')
+        else
+          tx = 0.0d0
+          txe = 0.0d0
+          txb = 0.0d0
+          flux = 0.0d0
+        end if
+        ierr = pm_get_state(ctx, u, u0, dp, poro, tnow)
+        call check(ierr, 'pm_get_state')
+        if (.not. allocated(ix20)) then
+          allocate(ix20(21 * d%numel), idn(4 * nnode), perm(nnode))
+          ierr = pm_get_numbering(ctx, ix20, idn, perm)
+          call check(ierr, 'pm_get_numbering')
+        end if
+        call write_res(res_name(prefix, istep, d%vtk_legacy), ix20, d%x, d%nnodev, u, dp, poro, &
+                       tx, txb, txe, flux, nnode, d%numel, istep, t, d%fprint, d%vtk_legacy, &
+                       d%vtk_binary, k)
+        if (k /= 0) then
+          write(*, '(a,i3)') ' *** pres: results file not written, code', k
+          stop 2
+        end if
       end select
     end do
     close(flog)

@@ -62,6 +62,15 @@ module pmhip
       integer(c_int) :: nnode, neq
       integer(c_int64_t) :: nad
     end function
+    integer(c_int) function pm_mesh_numbering(numel, nnodev, ix, idv, reorder, nnode_max, ix20, &
+                                              id, perm, nnode, neq, nad) &
+        bind(C, name='pm_mesh_numbering')
+      import :: c_int, c_int64_t
+      integer(c_int), value :: numel, nnodev, reorder, nnode_max
+      integer(c_int) :: ix(*), idv(*), ix20(*), id(*), perm(*)
+      integer(c_int) :: nnode, neq
+      integer(c_int64_t) :: nad
+    end function
     integer(c_int) function pm_get_numbering(ctx, ix20, id, perm) bind(C, name='pm_get_numbering')
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx

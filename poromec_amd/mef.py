@@ -19,7 +19,10 @@ the prefix of every output file).  Files written, named as ``name``
 * ``<prefix>_log_1.txt``: the solver log, one line per linear solve;
 * ``<prefix>_log_nonlinear_1.txt``: ``step istep t`` per ``solv`` macro
   (``Main.f:892``) and the nonlinear iterations it took;
-* ``<prefix>_t_1.txt``: device times and the sizes of the system.
+* ``<prefix>_t_1.txt``: device times and the sizes of the system;
+* for every ``pres``, ``<prefix>_istep_<n>.vtu`` (``.vtk`` with
+  ``legacy_vtk``): the mesh and the fields ``setprint`` selects
+  (``write_mesh_res_pm``, ``Write_vtk.f:912``; :mod:`poromec_amd.vtk`).
 
 Every step runs in the HIP library (``pm_step``); there is no CPU path.
 """
@@ -32,32 +35,11 @@ import time
 
 import numpy as np
 
+from . import vtk
 from .deck import MefInput, read_deck
+from .ftext import fortran_e
 from .problem import NDF
 from .solver import SOLVER_NAMES, SOLVERS, MefSolver
-
-
-def fortran_e(v: float, w: int = 18, d: int = 10) -> str:
-    """``v`` as the Fortran edit descriptor Ew.d writes it: ``0.ddddE+xx``,
-    the exponent without its letter when it needs three digits."""
-    v = float(v)
-    if v != v or v in (float("inf"), float("-inf")):
-        s = "NaN" if v != v else ("Infinity" if v > 0 else "-Infinity")
-        return s.rjust(w)
-    if v == 0.0:
-        body, sign = "0." + "0" * d + "E+00", "-" if np.signbit(v) else ""
-    else:
-        sign = "-" if v < 0 else ""
-        m, e = f"{abs(v):.{d - 1}e}".split("e")
-        exp = int(e) + 1
-        digits = m.replace(".", "")
-        if abs(exp) <= 99:
-            tail = f"E{'+' if exp >= 0 else '-'}{abs(exp):02d}"
-        else:
-            tail = f"{'+' if exp >= 0 else '-'}{abs(exp):03d}"
-        body = "0." + digits + tail
-    s = sign + body
-    return s.rjust(w) if len(s) <= w else "*" * w
 
 
 def printnode_line(istep: int, t: float, u) -> str:
@@ -109,6 +91,7 @@ def run(inp: MefInput, prefix: str, precision: str = "fp64", ordered: bool = Fal
         dt, t, istep = 1.0, 0.0, 0
         new_file = True
         new_sf = True
+        ix20 = None
         ms_elem = ms_solve = ms_total = 0.0
         with open(f"{prefix}_log_1.txt", "w") as flog, \
                 open(f"{prefix}_log_nonlinear_1.txt", "w") as fnl:
@@ -143,6 +126,20 @@ def run(inp: MefInput, prefix: str, precision: str = "fp64", ordered: bool = Fal
                                 f.write(printnode_line(istep, t, arr[n - 1]) + "\n")
                             shist[stem][n].append([istep, t, *arr[n - 1]])
                     new_sf = False
+                elif op[0] == "pres":
+                    fl = inp.print_flags
+                    st = s.state()
+                    if any(fl[4:8]):
+                        tx, txe, txb, flux = s.tform()
+                    else:
+                        tx = txe = txb = np.zeros((s.nnode, 6))
+                        flux = np.zeros((s.nnode, 3))
+                    if ix20 is None:
+                        ix20 = s.numbering()[0]
+                    vtk.write_results(vtk.results_name(prefix, istep, inp.vtk_legacy), ix20,
+                                      deck.x, st["u"], st["dp"], st["porosity"], tx, txb, txe,
+                                      flux, istep, t, flags=fl, quad=fl[0], legacy=inp.vtk_legacy,
+                                      binary=inp.vtk_binary, nnodev=deck.nnodev)
                 elif op[0] == "solv":
                     istep += 1
                     t = t + dt

@@ -16,7 +16,7 @@ from poromec_amd.solver import SOLVERS, SolverConfig
 
 ROOT = Path(__file__).resolve().parents[1]
 EXE = ROOT / "poromec_amd" / "fortran" / "pm_mefdrive"
-OPS = {1: "solv", 2: "pnup", 3: "pnsf", 4: "dt"}
+OPS = {1: "solv", 2: "pnup", 3: "pnsf", 4: "dt", 5: "pres"}
 
 pytestmark = pytest.mark.skipif(not EXE.exists(), reason="Fortran host not built (amdflang absent)")
 
@@ -61,6 +61,9 @@ def fparse(path, tmp):
     out["e"] = take(np.float64, 15 * numat).reshape(numat, 15)
     out["u0"] = take(np.float64, 4 * nnodev).reshape(nnodev, 4)
     out["fload"] = take(np.float64, 1000 * 5 * 200).reshape(200, 5, 1000)
+    fl = take(np.int32, 15)
+    out["print_flags"] = tuple(bool(v) for v in fl[:13])
+    out["vtk_legacy"], out["vtk_binary"] = bool(fl[13]), bool(fl[14])
     assert pos == len(b)
     return out, r.stdout
 
@@ -89,6 +92,8 @@ def assert_same(f, inp):
     assert np.array_equal(f["e"], d.e)
     assert np.array_equal(f["u0"], inp.u0)
     assert np.array_equal(f["fload"], d.fload)
+    assert f["print_flags"] == tuple(inp.print_flags)
+    assert (f["vtk_legacy"], f["vtk_binary"]) == (inp.vtk_legacy, inp.vtk_binary)
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -187,3 +192,44 @@ def test_conseq_without_elmt18_rejected(tmp_path):
     path.write_text("\n".join(["conseq cq.dat", "mesh", *_mesh_lines(1), "end mesh", "stop"]) + "\n")
     f, log = fparse(path, tmp_path)
     assert f is None and "Invalid elements" in log
+
+
+@pytest.mark.parametrize("before_mesh", [True, False])
+def test_setprint_pres_and_vtk_keys(tmp_path, before_mesh):
+    """setprint (first word per line until end, every flag cleared first;
+    pconsolidation and elplastic dropped at the mesh of an elastic run),
+    pres in the program, and the config keys bvtk and legacy_vtk."""
+    (tmp_path / "cfg.dat").write_text("reord true\nbvtk true\nlegacy_vtk true\nend\n")
+    words = ["totalstress", "darcyflux x", "porosity", "dpressure"]
+    if before_mesh:
+        words += ["quadratic", "pconsolidation"]
+    (tmp_path / "print.dat").write_text("\n".join(words) + "\nend\nbiotstress\n")
+    sp = ["setprint print.dat"]
+    L = ["config cfg.dat", *(sp if before_mesh else []), "mesh", *_mesh_lines(2), "end mesh",
+         *([] if before_mesh else sp), "setpnode node.dat", "pres", "loop 2", "solv", "pres",
+         "next", "stop"]
+    (tmp_path / "node.dat").write_text("1\n1\n")
+    path = tmp_path / "sp.dat"
+    path.write_text("\n".join(L) + "\n")
+    f, log = fparse(path, tmp_path)
+    assert f is not None, log
+    inp = read_deck(path)
+    fl = inp.print_flags
+    assert fl[0] == before_mesh and fl[4] and fl[7] and fl[8] and fl[3]
+    assert not (fl[1] or fl[2] or fl[5] or fl[9])
+    assert inp.vtk_legacy and inp.vtk_binary
+    assert [op[0] for op in inp.program].count("pres") == 3
+    assert_same(f, inp)
+
+
+@pytest.mark.parametrize("text,msg", [("quadratic\nend\n", "quadratic"),
+                                      ("elplastic\nend\n", "plastic path")])
+def test_setprint_after_mesh_rejected(tmp_path, text, msg):
+    (tmp_path / "print.dat").write_text(text)
+    path = tmp_path / "spx.dat"
+    path.write_text("\n".join(["mesh", *_mesh_lines(1), "end mesh", "setprint print.dat",
+                               "stop"]) + "\n")
+    f, log = fparse(path, tmp_path)
+    assert f is None and msg in log
+    with pytest.raises(Exception, match=msg):
+        read_deck(path)

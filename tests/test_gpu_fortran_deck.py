@@ -89,3 +89,50 @@ def test_fortran_host_vprop_deck(tmp_path):
     Path(path).write_text(text.replace("pnup\nnext", "pnup\npnsf\nnext"))
     inp, _ = _run_both(path, tmp_path, 0)
     assert inp.vprop[0] and (inp.ie == 18).all()
+
+
+@pytest.mark.parametrize("legacy,binary,quad", [(False, False, True), (True, False, False),
+                                                (True, True, True)])
+def test_fortran_host_results_files(tmp_path, legacy, binary, quad):
+    """pres through both hosts: every field setprint offers, over the
+    vertices or every node, in the three encodings; the same bytes (the VTU
+    time values: the same double), and the pressures and stresses in the
+    files are the printed histories' values to the 5 digits written."""
+    from poromec_amd import vtk as V
+    print(f"pres legacy={legacy} binary={binary} quad={quad}", flush=True)
+    deck = P.column_deck(2, 2, 5, dt=0.05, load_mpa=0.5)
+    path = write_deck(tmp_path / "r.dat", deck, SolverConfig(stop_crit=1), pnodes=(1, 9),
+                      nsteps=3)
+    (tmp_path / "r_config.dat").write_text(
+        f"reord true\nlegacy_vtk {str(legacy).lower()}\nbvtk {str(binary).lower()}\nend\n")
+    words = [w for w in V.PRINT_WORDS[1:9]] + (["quadratic"] if quad else [])
+    (tmp_path / "r_print.dat").write_text("\n".join(words) + "\nend\n")
+    text = Path(path).read_text().replace("mesh\n", "setprint r_print.dat\nmesh\n", 1)
+    text = text.replace("pnup\nnext", "pnup\npnsf\npres\nnext")
+    Path(path).write_text(text)
+    inp, fo = _run_both(path, tmp_path, 0)
+    assert inp.print_flags[0] == quad and inp.vtk_legacy == legacy
+    py = str(tmp_path / "py")
+    for istep in (1, 2, 3):
+        a = Path(V.results_name(py, istep, legacy)).read_bytes()
+        b = Path(V.results_name(fo, istep, legacy)).read_bytes()
+        if not legacy:
+            la, lb = a.split(b"\n"), b.split(b"\n")
+            for j in range(len(la)):
+                if b'Name="TIME_' in la[j]:
+                    assert float(la[j + 1]) == float(lb[j + 1])
+                    la[j + 1] = lb[j + 1]
+            a = b"\n".join(la)
+        assert a == b, istep
+    if legacy and not binary:
+        L = Path(V.results_name(py, 3, True)).read_text().split("\n")
+        i = L.index(V._scalar_head("pressao".ljust(15), "double ")) + 2
+        p = np.array([float(v) for v in L[i:i + deck.nnodev]])
+        j = L.index("TENSORS " + "totalStress".ljust(15) + " double") + 1
+        sx = np.array([[float(v) for v in ln.split()] for ln in L[j:j + deck.nnodev]])
+        for n in inp.pnodes:
+            _, _, up = read_printnode(f"{py}_up_node_{n}.txt")
+            assert np.isclose(p[n - 1], up[-1, 3], rtol=1e-4, atol=1e-9 * abs(up[:, 3]).max())
+            _, _, st = read_printnode(f"{py}_stress_node_{n}.txt")
+            full = st[-1][[0, 3, 5, 3, 1, 4, 5, 4, 2]]
+            assert np.allclose(sx[n - 1], full, rtol=1e-4, atol=1e-6 * abs(st).max())
