@@ -31,9 +31,19 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "time-steps/s at 8.6M-dof Biot u-p (FP32+IR vs FP64 @1e-10); SpMV GB/s"
-# (lanes per node block, columns in flight per lane) of the SpMV shapes 0..5
-shape_lanes = (64, 64, 64, 64, 64, 32)
-shape_unr = (1, 2, 3, 4, 5, 4)
+# (lanes per node block, columns in flight per lane, index prefetch) of the
+# SpMV shapes 0..8
+shape_lanes = (64, 64, 64, 64, 64, 32, 64, 64, 64)
+shape_unr = (1, 2, 3, 4, 5, 4, 3, 4, 5)
+shape_pf = (0, 0, 0, 0, 0, 0, 1, 1, 1)
+
+
+def spmv_kernel_name(prec, shape):
+    """The SpMV kernel the solver launches at this shape, with its fused dot."""
+    vt = "float" if prec == "fp32" else "double"
+    if shape_pf[shape]:
+        return f"k_spmv_pf<{vt}, {shape_unr[shape]}, 1, false>"
+    return f"k_spmv<{vt}, {shape_lanes[shape]}, {shape_unr[shape]}, 1, false>"
 HBM_PEAK_GBPS = 8000.0
 
 
@@ -515,9 +525,8 @@ def main():
                  "tuned_shapes": shapes,
                  "csr_equivalent_fp64_GBps": csr_bytes(info) / ms64 / 1e6},
         "roofline": {"bound": "hbm",
-                     "kernel": f"k_spmv<float, {shape_lanes[shapes['fp32']['best']]}, "
-                               f"{shape_unr[shapes['fp32']['best']]}, 1, false> "
-                               "(FP32 node-block SpMV with the fused q.t dot)",
+                     "kernel": spmv_kernel_name("fp32", shapes["fp32"]["best"])
+                               + " (FP32 node-block SpMV with the fused q.t dot)",
                      "achieved": gbps32, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": gbps32 / HBM_PEAK_GBPS,
                      "traffic": measured_traffic("fp32", shapes["fp32"]["best"], info)},

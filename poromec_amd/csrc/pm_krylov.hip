@@ -78,6 +78,66 @@ __global__ __launch_bounds__(PM_WG) void k_spmv(
   }
 }
 
+// The same product with each block's first-pass column indices loaded one
+// block ahead (pm_spmv_block64_pf).  A wavefront's blocks k, k + GROUPS, ...
+// are wave-uniform (grp via readfirstlane), so their metadata loads are
+// scalar loads, two blocks ahead.
+template <typename VT, int UNR, int NV, bool SCALE_M>
+__global__ __launch_bounds__(PM_WG) void k_spmv_pf(
+    int nrn, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
+    const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
+    const int *__restrict__ ncol, const VT *__restrict__ val, const double *__restrict__ x,
+    double *__restrict__ y, const double *__restrict__ w, const double *__restrict__ m,
+    double *partials, const PmScalars *S) {
+  if (S != nullptr && S->done) return;
+  constexpr int GROUPS = PM_WG / 64;
+  const __amdgpu_buffer_rsrc_t xr = pm_rsrc(x, (unsigned int)neq * 8u);
+  const int lane = threadIdx.x % 64;
+  const int grp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+  const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
+  int64_t k0, k1;
+  pm_chunk(nrn, lb, gridDim.x, k0, k1);
+  double d0 = 0.0, d1 = 0.0;
+  PmBlockMeta cur = pm_uniform<64>(pm_block_meta(k0 + grp, k1, colptr, valptr, nr));
+  PmBlockMeta nx = pm_uniform<64>(pm_block_meta(k0 + grp + GROUPS, k1, colptr, valptr, nr));
+  unsigned int col[UNR];
+  {
+    const __amdgpu_buffer_rsrc_t c0 = pm_rsrc(ncol + cur.cp, (unsigned int)cur.c * 4u);
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      col[u] = __builtin_amdgcn_raw_buffer_load_b32(c0, (unsigned int)(lane + u * 64) * 4u, 0, 2);
+  }
+  for (int64_t k = k0 + grp; k < k1; k += GROUPS) {
+    const PmBlockMeta nn = pm_uniform<64>(pm_block_meta(k + 2 * GROUPS, k1, colptr, valptr, nr));
+    const int n = cur.n;
+    const int f = fr[k];
+    double wq[4] = {0.0, 0.0, 0.0, 0.0}, mq[4] = {1.0, 1.0, 1.0, 1.0};
+    if (lane == 63) {
+      for (int q = 0; q < n; q++) {
+        if (NV >= 1) wq[q] = w[f + q];
+        if (SCALE_M) mq[q] = m[f + q];
+      }
+    }
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    pm_spmv_block64_pf<VT, UNR>(lane, cur, nx, col, ncol, val, xr, a);
+    if (lane == 63) {
+      for (int q = 0; q < n; q++) {
+        double yq = a[q];
+        if (SCALE_M) yq *= mq[q];
+        y[f + q] = yq;
+        if (NV >= 1) d0 += yq * wq[q];
+        if (NV >= 2) d1 += yq * yq;
+      }
+    }
+    cur = nx;
+    nx = nn;
+  }
+  if (NV > 0) {
+    double v[2] = {d0, d1};
+    pm_block_partials<NV>(v, partials);
+  }
+}
+
 // Sums the workgroup partials of the kernel before it (fixed order) into
 // S->red and runs the epilogue that computes the next solver scalars.
 template <int NV, class Epi>
@@ -118,15 +178,25 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
                        A.nrn, A.neq, A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w, m,   \
                        c->d_partials, S);                                                       \
     break;
+#define PM_SPMV_PF_CASE(id, U)                                                                  \
+  case id:                                                                                      \
+    hipLaunchKernelGGL((k_spmv_pf<VT, U, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream,    \
+                       A.nrn, A.neq, A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w, m,   \
+                       c->d_partials, S);                                                       \
+    break;
   switch (cfg) {
     PM_SPMV_CASE(0, 64, 1)
     PM_SPMV_CASE(1, 64, 2)
     PM_SPMV_CASE(2, 64, 3)
     PM_SPMV_CASE(3, 64, 4)
     PM_SPMV_CASE(4, 64, 5)
+    PM_SPMV_PF_CASE(6, 3)
+    PM_SPMV_PF_CASE(7, 4)
+    PM_SPMV_PF_CASE(8, 5)
     default: PM_SPMV_CASE(5, 32, 4)
   }
 #undef PM_SPMV_CASE
+#undef PM_SPMV_PF_CASE
   return g;
 }
 

@@ -12,10 +12,12 @@
 #pragma once
 #include "pm_device.h"
 
-// (LANES, UNR) of each SpMV shape the tuner may pick
-#define PM_SPMV_NCFG 6
-static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 64, 64, 32};
-static constexpr int pm_spmv_unr[PM_SPMV_NCFG] = {1, 2, 3, 4, 5, 4};
+// (LANES, UNR, PF) of each SpMV shape the tuner may pick; PF: the first
+// pass's column indices are loaded one block ahead (pm_spmv_block64_pf)
+#define PM_SPMV_NCFG 9
+static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 64, 64, 32, 64, 64, 64};
+static constexpr int pm_spmv_unr[PM_SPMV_NCFG] = {1, 2, 3, 4, 5, 4, 3, 4, 5};
+static constexpr int pm_spmv_pf[PM_SPMV_NCFG] = {0, 0, 0, 0, 0, 0, 1, 1, 1};
 
 struct PmBlockMeta {
   int64_t cp, vp;
@@ -132,6 +134,85 @@ __device__ __forceinline__ void pm_spmv_block64(int lane, const PmBlockMeta &m,
       // past the end of the block the column reads as 0 and the values as
       // the next row's: keep x out of those lanes
       const double xc = pm_bload<double, 0>(xr, col[u] * 8u);
+      xv[u] = base + lane + u * 64 < c ? xc : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      a[0] += (double)w0[u] * xv[u];
+      a[1] += (double)w1[u] * xv[u];
+      a[2] += (double)w2[u] * xv[u];
+      a[3] += (double)w3[u] * xv[u];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) a[q] = pm_group_sum<64>(a[q]);
+}
+
+// The 64-lane block with its first pass's column indices already in hand:
+// on entry col[] holds block m's, on return block mn's (the next block of
+// this wavefront).  Without them the x gathers wait one memory round trip
+// for the index loads; with them the first pass issues its value loads and
+// its gathers together, then the next block's index loads, and its sums wait
+// only for the first two (loads complete in order, so the wait leaves the
+// index loads in flight).  Buffer resources as in pm_spmv_block64: loads
+// past a block's end, or of an empty next block, read zero.
+template <typename VT, int UNR>
+__device__ __forceinline__ void pm_spmv_block64_pf(int lane, const PmBlockMeta &m,
+                                                   const PmBlockMeta &mn, unsigned int (&col)[UNR],
+                                                   const int *__restrict__ ncol,
+                                                   const VT *__restrict__ val,
+                                                   __amdgpu_buffer_rsrc_t xr, double a[4]) {
+  const int c = m.c, n = m.n;
+  const __amdgpu_buffer_rsrc_t cr = pm_rsrc(ncol + m.cp, (unsigned int)c * 4u);
+  const __amdgpu_buffer_rsrc_t vr =
+      pm_rsrc(val + m.vp, (unsigned int)n * (unsigned int)c * (unsigned int)sizeof(VT));
+  const __amdgpu_buffer_rsrc_t crn = pm_rsrc(ncol + mn.cp, (unsigned int)mn.c * 4u);
+  const unsigned int rowb = (unsigned int)c * (unsigned int)sizeof(VT);
+  constexpr int NT = 2;  // non-temporal: the matrix streams through once
+  {
+    VT w0[UNR], w1[UNR], w2[UNR], w3[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const unsigned int o = (unsigned int)(lane + u * 64) * (unsigned int)sizeof(VT);
+      w0[u] = pm_bload<VT, NT>(vr, o);
+      w1[u] = pm_bload<VT, NT>(vr, o + rowb);
+      w2[u] = pm_bload<VT, NT>(vr, o + 2u * rowb);
+      w3[u] = pm_bload<VT, NT>(vr, o + 3u * rowb);
+    }
+    double xv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const double xc = pm_bload<double, 0>(xr, col[u] * 8u);
+      xv[u] = lane + u * 64 < c ? xc : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      col[u] = __builtin_amdgcn_raw_buffer_load_b32(crn, (unsigned int)(lane + u * 64) * 4u, 0, NT);
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      a[0] += (double)w0[u] * xv[u];
+      a[1] += (double)w1[u] * xv[u];
+      a[2] += (double)w2[u] * xv[u];
+      a[3] += (double)w3[u] * xv[u];
+    }
+  }
+  for (int base = UNR * 64; base < c; base += UNR * 64) {
+    unsigned int cl[UNR];
+    VT w0[UNR], w1[UNR], w2[UNR], w3[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const unsigned int j = (unsigned int)(base + lane + u * 64);
+      cl[u] = __builtin_amdgcn_raw_buffer_load_b32(cr, j * 4u, 0, NT);
+      const unsigned int o = j * (unsigned int)sizeof(VT);
+      w0[u] = pm_bload<VT, NT>(vr, o);
+      w1[u] = pm_bload<VT, NT>(vr, o + rowb);
+      w2[u] = pm_bload<VT, NT>(vr, o + 2u * rowb);
+      w3[u] = pm_bload<VT, NT>(vr, o + 3u * rowb);
+    }
+    double xv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const double xc = pm_bload<double, 0>(xr, cl[u] * 8u);
       xv[u] = base + lane + u * 64 < c ? xc : 0.0;
     }
 #pragma unroll

@@ -137,7 +137,7 @@ def test_fp32ir_reaches_tolerance(column):
 
 
 def test_fp32ir_every_spmv_shape():
-    """Every SpMV shape, the software-pipelined ones included, on a box whose
+    """Every SpMV shape, the index-prefetch ones included, on a box whose
     interior node blocks are wider than one pass of the narrower shapes
     (270 columns): FP32+IR reaches the FP64 solution."""
     from poromec_amd.solver import SPMV_SHAPES, SolverConfig
@@ -152,6 +152,27 @@ def test_fp32ir_every_spmv_shape():
         x32, i32 = s.solve(b, SolverConfig(tol=1e-10, precision="fp32ir"))
         assert i32["true_rel"] < 1e-10, shape
         assert np.abs(x32 - x64).max() <= 1e-8 * np.abs(x64).max(), shape
+
+
+def test_fp64_spmv_every_shape():
+    """Every FP64 SpMV shape, the index-prefetch ones included, against the
+    exported CSRC matrix applied on the host (A = diag + L + L^T), on a box
+    whose interior node blocks need two passes of the (64, 3) shape."""
+    import scipy.sparse as sp
+    from poromec_amd.solver import SPMV_SHAPES
+    d = P.heterogeneous_box_deck(4, seed=3, sigma_lnk=0.5, corr_len=2.0)
+    s = _solver(d)
+    s.pform(2, True, False, d.dt, d.dt)
+    ia, ja, ad, al = s.export_csrc()
+    n = s.neq
+    rows = np.repeat(np.arange(n), np.diff(ia))
+    L = sp.csr_matrix((al, (rows, ja - 1)), shape=(n, n))
+    x = np.random.default_rng(8).standard_normal(n)
+    y0 = ad * x + L @ x + L.T @ x
+    for shape in range(SPMV_SHAPES):
+        s.set_spmv_shape(False, shape)
+        y = s.spmv(x)
+        assert np.abs(y - y0).max() <= 1e-13 * np.abs(y0).max(), shape
 
 
 def test_csrc_dropin_solve(column):
