@@ -31,16 +31,18 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "time-steps/s at 8.6M-dof Biot u-p (FP32+IR vs FP64 @1e-10); SpMV GB/s"
-# (lanes per node block, columns in flight per lane, index prefetch) of the
-# SpMV shapes 0..8
-shape_lanes = (64, 64, 64, 64, 64, 32, 64, 64, 64)
-shape_unr = (1, 2, 3, 4, 5, 4, 3, 4, 5)
-shape_pf = (0, 0, 0, 0, 0, 0, 1, 1, 1)
+# (lanes per node block, columns in flight per lane, index prefetch / runs)
+# of the SpMV shapes 0..10
+shape_lanes = (64, 64, 64, 64, 64, 32, 64, 64, 64, 64, 64)
+shape_unr = (1, 2, 3, 4, 5, 4, 3, 4, 5, 1, 2)
+shape_pf = (0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2)
 
 
 def spmv_kernel_name(prec, shape):
     """The SpMV kernel the solver launches at this shape, with its fused dot."""
     vt = "float" if prec == "fp32" else "double"
+    if shape_pf[shape] == 2:
+        return f"k_spmv_ch<{vt}, {shape_unr[shape]}, 1, false>"
     if shape_pf[shape]:
         return f"k_spmv_pf<{vt}, {shape_unr[shape]}, 1, false>"
     return f"k_spmv<{vt}, {shape_lanes[shape]}, {shape_unr[shape]}, 1, false>"
@@ -51,13 +53,18 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def spmv_bytes(info, value_bytes, fused_dot=False):
+def spmv_bytes(info, value_bytes, fused_dot=False, shape=None):
     """Bytes one SpMV launch must move in the node-block layout: every value
-    once, every shared column index once, the per-block metadata (colptr,
-    valptr: 8 B each; first row, row count: 4 B each), x once, y once, and w
-    once when the solver's dot y.w is fused into it."""
-    return (value_bytes * info["nnz"] + 4 * info["ncols"] + 24 * info["nblocks"]
-            + (24 if fused_dot else 16) * info["neq"])
+    once, the column structure once, the per-block metadata, x once, y once,
+    and w once when the solver's dot y.w is fused into it.  The column
+    structure is one shared 32-bit index per column (colptr, valptr: 8 B
+    each, first row and row count: 4 B each per block), or for the
+    run-per-lane shapes one 32-bit word per run of <= 4 consecutive columns
+    (plus chptr: 8 B per block)."""
+    runs = shape is not None and shape_pf[shape] == 2
+    cols = 4 * info["nruns"] if runs else 4 * info["ncols"]
+    meta = (32 if runs else 24) * info["nblocks"]
+    return value_bytes * info["nnz"] + cols + meta + (24 if fused_dot else 16) * info["neq"]
 
 
 def csr_bytes(info):
@@ -429,8 +436,9 @@ def main():
         copy_bytes = 1 << 30
         msc = solver.bench_copy(copy_bytes, args.spmv_reps)
         print(json.dumps({"spmv_fp64_ms": ms64, "spmv_fp32_ms": ms32,
-                          "spmv_fp64_bytes": spmv_bytes(info, 8),
-                          "spmv_fp32_bytes": spmv_bytes(info, 4), "copy_ms": msc,
+                          "spmv_fp64_bytes": spmv_bytes(info, 8, shape=shapes["fp64"]["best"]),
+                          "spmv_fp32_bytes": spmv_bytes(info, 4, shape=shapes["fp32"]["best"]),
+                          "copy_ms": msc,
                           "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes,
                           "copy_GBps": 2 * copy_bytes / msc / 1e6, "tuned_shapes": shapes,
                           **info}), flush=True)
@@ -463,7 +471,8 @@ def main():
 
     # SpMV kernel time inside the timed steps (HIP events on the library's
     # stream around every solver SpMV), per value precision
-    b64, b32 = spmv_bytes(info, 8, fused_dot=True), spmv_bytes(info, 4, fused_dot=True)
+    b64 = spmv_bytes(info, 8, fused_dot=True, shape=shapes["fp64"]["best"])
+    b32 = spmv_bytes(info, 4, fused_dot=True, shape=shapes["fp32"]["best"])
     ms64 = live64["fp64_ms"] / max(live64["fp64_n"], 1)
     ms32 = live32["fp32_ms"] / max(live32["fp32_n"], 1)
     gbps64, gbps32 = b64 / ms64 / 1e6, b32 / ms32 / 1e6

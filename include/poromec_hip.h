@@ -251,6 +251,9 @@ int pm_adjoint(pm_ctx *ctx, int nsteps, double dt, const pm_solver_cfg *cfg, int
  * Sizes of the device matrix: equations, row blocks (free nodes), stored
  * values (full matrix) and shared column indices. */
 int pm_matrix_info(pm_ctx *ctx, int *neq, int *nblocks, int64_t *nnz, int64_t *ncols);
+/* Runs of <= 4 consecutive columns the run-per-lane SpMV shapes read in
+ * place of the column indices (one 32-bit word each). */
+int pm_matrix_runs(pm_ctx *ctx, int64_t *nruns);
 /* Average device time of one SpMV launch (FP64 values, or FP32 values when
  * use32 != 0) over reps launches, timed with HIP events on the library's
  * stream. */
@@ -262,7 +265,7 @@ int pm_bench_spmv(pm_ctx *ctx, int use32, int reps, double *ms_per_launch);
  * not NULL, every shape's average launch time.  The batching form tunes its
  * own kernel.  Shapes sum in different orders: results move at the rounding
  * level when the shape changes, and stay fixed until the next change. */
-#define PM_SPMV_SHAPES 9
+#define PM_SPMV_SHAPES 11
 int pm_tune_spmv(pm_ctx *ctx, int use32, int reps, int *best, double *ms_per_shape);
 /* Selects SpMV shape `shape` (0 .. PM_SPMV_SHAPES-1) without timing. */
 int pm_set_spmv_shape(pm_ctx *ctx, int use32, int shape);

@@ -188,7 +188,7 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
                        y, w,                                                                    \
                        B->d_S);                                                                 \
     break;
-  switch (cfg) {  // the batched kernel has no index-prefetch form: 6-8 run as 2-4
+  switch (cfg) {  // the batched kernel has neither the prefetch nor the run form: 6-10 run as 2-4
     PM_BSPMV_CASE(0, 64, 1)
     PM_BSPMV_CASE(1, 64, 2)
     PM_BSPMV_CASE(2, 64, 3)
@@ -197,6 +197,8 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
     PM_BSPMV_CASE(6, 64, 3)
     PM_BSPMV_CASE(7, 64, 4)
     PM_BSPMV_CASE(8, 64, 5)
+    PM_BSPMV_CASE(9, 64, 3)
+    PM_BSPMV_CASE(10, 64, 4)
     default: PM_BSPMV_CASE(5, 32, 4)
   }
 #undef PM_BSPMV_CASE

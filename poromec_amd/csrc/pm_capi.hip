@@ -45,6 +45,8 @@ static void free_matrix(PmDevMatrix &A) {
   FREE(A.eq_block);
   FREE(A.colptr);
   FREE(A.valptr);
+  FREE(A.chptr);
+  FREE(A.chunk);
   FREE(A.val);
   FREE(A.val32);
   FREE(A.diag);
@@ -178,8 +180,31 @@ int pm_upload_pattern(pm_ctx *c, const PmHostPattern &p) {
   PM_TRY(dupload(c, &A.valptr, p.valptr.data(), p.valptr.size()));
   PM_TRY(dupload(c, &A.ncol, p.ncol.data(), p.ncol.size()));
   PM_TRY(dupload(c, &A.eq_block, p.eq_block.data(), p.eq_block.size()));
-  PM_TRY(dalloc(&A.val, (size_t)A.nnz));
-  PM_CHECK_HIP(hipMemsetAsync(A.val, 0, sizeof(double) * (size_t)A.nnz, c->stream));
+  {
+    // runs of consecutive columns, cut at 4, for the run-per-lane SpMV
+    std::vector<int64_t> chptr((size_t)A.nrn + 1);
+    std::vector<unsigned int> chunk;
+    chunk.reserve((size_t)(A.ncols / 3 + A.nrn));
+    for (int k = 0; k < A.nrn; k++) {
+      chptr[k] = (int64_t)chunk.size();
+      int64_t j = p.colptr[k];
+      const int64_t e = p.colptr[k + 1];
+      while (j < e) {
+        const unsigned int s0 = (unsigned int)p.ncol[j];
+        int len = 1;
+        while (len < 4 && j + len < e && (unsigned int)p.ncol[j + len] == s0 + (unsigned int)len) len++;
+        chunk.push_back((s0 << 2) | (unsigned int)(len - 1));
+        j += len;
+      }
+    }
+    chptr[A.nrn] = (int64_t)chunk.size();
+    A.nchunks = (int64_t)chunk.size();
+    PM_TRY(dupload(c, &A.chptr, chptr.data(), chptr.size()));
+    PM_TRY(dupload(c, &A.chunk, chunk.data(), chunk.size()));
+  }
+  // 4 values of padding: the run-per-lane SpMV reads a whole 4-value run
+  PM_TRY(dalloc(&A.val, (size_t)A.nnz + 4));
+  PM_CHECK_HIP(hipMemsetAsync(A.val, 0, sizeof(double) * ((size_t)A.nnz + 4), c->stream));
   PM_TRY(dalloc(&A.diag, (size_t)A.neq));
   PM_TRY(dalloc(&A.diagpos, (size_t)A.neq));
   PM_CHECK_HIP(hipStreamSynchronize(c->stream));
@@ -229,7 +254,10 @@ __global__ void k_to32(int64_t n, const double *a, float *b) {
 
 int pm_make_val32(pm_ctx *c) {
   PmDevMatrix &A = c->A;
-  if (!A.val32) PM_TRY(dalloc(&A.val32, (size_t)A.nnz));
+  if (!A.val32) {  // padded as val
+    PM_TRY(dalloc(&A.val32, (size_t)A.nnz + 4));
+    PM_CHECK_HIP(hipMemsetAsync(A.val32 + A.nnz, 0, 4 * sizeof(float), c->stream));
+  }
   hipLaunchKernelGGL(k_to32, dim3(4096), dim3(256), 0, c->stream, A.nnz, A.val, A.val32);
   PM_CHECK_HIP(hipGetLastError());
   A.have32 = true;
@@ -887,6 +915,13 @@ extern "C" int pm_matrix_info(pm_ctx *c, int *neq, int *nblocks, int64_t *nnz, i
   if (nblocks) *nblocks = c->A.nrn;
   if (nnz) *nnz = c->A.nnz;
   if (ncols) *ncols = c->A.ncols;
+  return PM_OK;
+}
+
+extern "C" int pm_matrix_runs(pm_ctx *c, int64_t *nruns) {
+  if (!c || !nruns) return PM_ERR_ARG;
+  if (!c->have_matrix) return PM_ERR_STATE;
+  *nruns = c->A.nchunks;
   return PM_OK;
 }
 

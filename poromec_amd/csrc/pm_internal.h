@@ -99,6 +99,11 @@ struct PmDevMatrix {
   int64_t nnz = 0, ncols = 0;
   int *fr = nullptr, *nr = nullptr, *ncol = nullptr, *eq_block = nullptr;
   int64_t *colptr = nullptr, *valptr = nullptr;
+  // column runs of <= 4 consecutive columns per block: chunk[] holds
+  // (first column << 2) | (length - 1), chptr[] each block's first run
+  int64_t nchunks = 0;
+  int64_t *chptr = nullptr;
+  unsigned int *chunk = nullptr;
   double *val = nullptr;
   float *val32 = nullptr;
   double *diag = nullptr;  // ad

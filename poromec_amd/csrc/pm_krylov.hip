@@ -138,6 +138,180 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_pf(
   }
 }
 
+// The product with one run of <= 4 consecutive columns per lane (chunk[],
+// chptr[]): a lane loads its run's 32-bit descriptor, the run's 4 values of
+// each row as one 16-byte load (two for FP64), and x at the run's columns.
+// That is ~3.1 columns per lane for one index word instead of one, and half
+// the load instructions per column.  A lane's position in the block's
+// column list is the sum of the runs before it, taken from two ballots of
+// the length bits (mbcnt), with no LDS.  The descriptors of a block's first
+// pass are loaded one block ahead, as k_spmv_pf does with column indices.
+// Loads past a run read the next columns' values (or the 4 values of padding
+// after the array) and are multiplied by zero; rows past the block's count
+// accumulate into sums that are not stored.
+struct PmRunMeta {
+  int64_t vp, hp;
+  int c, n, nh;
+};
+
+__device__ __forceinline__ PmRunMeta pm_run_meta(int64_t k, int64_t k1,
+                                                 const int64_t *__restrict__ colptr,
+                                                 const int64_t *__restrict__ valptr,
+                                                 const int *__restrict__ nr,
+                                                 const int64_t *__restrict__ chptr) {
+  PmRunMeta m{0, 0, 0, 0, 0};
+  if (k < k1) {
+    m.vp = valptr[k];
+    m.c = (int)(colptr[k + 1] - colptr[k]);
+    m.n = nr[k];
+    m.hp = chptr[k];
+    m.nh = (int)(chptr[k + 1] - m.hp);
+  }
+  return m;
+}
+
+__device__ __forceinline__ unsigned int pm_lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned int)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
+}
+
+// one pass of UNR runs per lane over block m: d[] the descriptors, base the
+// block columns before the pass; returns the columns the pass covered
+template <typename VT, int UNR, bool PREFETCH>
+__device__ __forceinline__ unsigned int pm_run_pass(int lane, int h0, const PmRunMeta &m,
+                                                    unsigned int base, const unsigned int (&d)[UNR],
+                                                    __amdgpu_buffer_rsrc_t vr, unsigned int rowb,
+                                                    __amdgpu_buffer_rsrc_t xr,
+                                                    __amdgpu_buffer_rsrc_t hrn,
+                                                    unsigned int (&dn)[UNR], double a[4]) {
+  constexpr int NT = 2;
+  constexpr int VPR = sizeof(VT) == 4 ? 1 : 2;  // 16-byte loads per row and run
+  unsigned int off[UNR];
+  int len[UNR];
+#pragma unroll
+  for (int u = 0; u < UNR; u++) {
+    const bool valid = h0 + lane + u * 64 < m.nh;
+    const unsigned int l1 = d[u] & 3u;
+    const unsigned long long bv = __ballot(valid);
+    const unsigned long long b0 = __ballot(valid && (l1 & 1u));
+    const unsigned long long b1 = __ballot(valid && (l1 & 2u));
+    off[u] = base + pm_lanes_below(bv) + pm_lanes_below(b0) + 2u * pm_lanes_below(b1);
+    len[u] = valid ? (int)l1 + 1 : 0;
+    base += (unsigned int)(__popcll(bv) + __popcll(b0) + 2 * __popcll(b1));
+  }
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  u4 w[UNR][4][VPR];
+#pragma unroll
+  for (int u = 0; u < UNR; u++) {
+    const unsigned int o = off[u] * (unsigned int)sizeof(VT);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int h = 0; h < VPR; h++)
+        w[u][q][h] = __builtin_bit_cast(
+            u4, __builtin_amdgcn_raw_buffer_load_b128(vr, o + (unsigned int)q * rowb + 16u * h, 0, NT));
+  }
+  double xv[UNR][4];
+#pragma unroll
+  for (int u = 0; u < UNR; u++) {
+    const unsigned int col = d[u] >> 2;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const double xc = pm_bload<double, 0>(xr, (col + (unsigned int)k) * 8u);
+      xv[u][k] = k < len[u] ? xc : 0.0;
+    }
+  }
+  if (PREFETCH) {
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      dn[u] = __builtin_amdgcn_raw_buffer_load_b32(hrn, (unsigned int)(lane + u * 64) * 4u, 0, NT);
+  }
+#pragma unroll
+  for (int u = 0; u < UNR; u++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      VT v[4];
+      __builtin_memcpy(v, &w[u][q][0], sizeof(v));
+#pragma unroll
+      for (int k = 0; k < 4; k++) a[q] += (double)v[k] * xv[u][k];
+    }
+  return base;
+}
+
+template <typename VT, int UNR, int NV, bool SCALE_M>
+__global__ __launch_bounds__(PM_WG) void k_spmv_ch(
+    int nrn, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
+    const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
+    const int64_t *__restrict__ chptr, const unsigned int *__restrict__ chunk,
+    const VT *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
+    const double *__restrict__ w, const double *__restrict__ m, double *partials,
+    const PmScalars *S) {
+  if (S != nullptr && S->done) return;
+  constexpr int GROUPS = PM_WG / 64;
+  const __amdgpu_buffer_rsrc_t xr = pm_rsrc(x, (unsigned int)neq * 8u);
+  const int lane = threadIdx.x % 64;
+  const int grp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+  const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
+  int64_t k0, k1;
+  pm_chunk(nrn, lb, gridDim.x, k0, k1);
+  double d0 = 0.0, d1 = 0.0;
+  PmRunMeta cur = pm_run_meta(k0 + grp, k1, colptr, valptr, nr, chptr);
+  PmRunMeta nx = pm_run_meta(k0 + grp + GROUPS, k1, colptr, valptr, nr, chptr);
+  unsigned int dsc[UNR];
+  {
+    const __amdgpu_buffer_rsrc_t h0 = pm_rsrc(chunk + cur.hp, (unsigned int)cur.nh * 4u);
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      dsc[u] = __builtin_amdgcn_raw_buffer_load_b32(h0, (unsigned int)(lane + u * 64) * 4u, 0, 2);
+  }
+  for (int64_t k = k0 + grp; k < k1; k += GROUPS) {
+    const PmRunMeta nn = pm_run_meta(k + 2 * GROUPS, k1, colptr, valptr, nr, chptr);
+    const int n = cur.n;
+    const int f = fr[k];
+    double wq[4] = {0.0, 0.0, 0.0, 0.0}, mq[4] = {1.0, 1.0, 1.0, 1.0};
+    if (lane == 63) {
+      for (int q = 0; q < n; q++) {
+        if (NV >= 1) wq[q] = w[f + q];
+        if (SCALE_M) mq[q] = m[f + q];
+      }
+    }
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    const __amdgpu_buffer_rsrc_t vr = pm_rsrc(
+        val + cur.vp, ((unsigned int)n * (unsigned int)cur.c + 3u) * (unsigned int)sizeof(VT));
+    const unsigned int rowb = (unsigned int)cur.c * (unsigned int)sizeof(VT);
+    const __amdgpu_buffer_rsrc_t hr = pm_rsrc(chunk + cur.hp, (unsigned int)cur.nh * 4u);
+    const __amdgpu_buffer_rsrc_t hrn = pm_rsrc(chunk + nx.hp, (unsigned int)nx.nh * 4u);
+    unsigned int dn[UNR];
+    unsigned int base = pm_run_pass<VT, UNR, true>(lane, 0, cur, 0u, dsc, vr, rowb, xr, hrn, dn, a);
+    for (int h0 = UNR * 64; h0 < cur.nh; h0 += UNR * 64) {
+      unsigned int dp[UNR], unused[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++)
+        dp[u] = __builtin_amdgcn_raw_buffer_load_b32(hr, (unsigned int)(h0 + lane + u * 64) * 4u, 0, 2);
+      base = pm_run_pass<VT, UNR, false>(lane, h0, cur, base, dp, vr, rowb, xr, hrn, unused, a);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) dsc[u] = dn[u];
+#pragma unroll
+    for (int q = 0; q < 4; q++) a[q] = pm_group_sum<64>(a[q]);
+    if (lane == 63) {
+      for (int q = 0; q < n; q++) {
+        double yq = a[q];
+        if (SCALE_M) yq *= mq[q];
+        y[f + q] = yq;
+        if (NV >= 1) d0 += yq * wq[q];
+        if (NV >= 2) d1 += yq * yq;
+      }
+    }
+    cur = nx;
+    nx = nn;
+  }
+  if (NV > 0) {
+    double v[2] = {d0, d1};
+    pm_block_partials<NV>(v, partials);
+  }
+}
+
 // Sums the workgroup partials of the kernel before it (fixed order) into
 // S->red and runs the epilogue that computes the next solver scalars.
 template <int NV, class Epi>
@@ -184,6 +358,12 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
                        A.nrn, A.neq, A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w, m,   \
                        c->d_partials, S);                                                       \
     break;
+#define PM_SPMV_CH_CASE(id, U)                                                                  \
+  case id:                                                                                      \
+    hipLaunchKernelGGL((k_spmv_ch<VT, U, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream,    \
+                       A.nrn, A.neq, A.fr, A.nr, A.colptr, A.valptr, A.chptr, A.chunk, val, x,  \
+                       y, w, m, c->d_partials, S);                                              \
+    break;
   switch (cfg) {
     PM_SPMV_CASE(0, 64, 1)
     PM_SPMV_CASE(1, 64, 2)
@@ -193,10 +373,13 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
     PM_SPMV_PF_CASE(6, 3)
     PM_SPMV_PF_CASE(7, 4)
     PM_SPMV_PF_CASE(8, 5)
+    PM_SPMV_CH_CASE(9, 1)
+    PM_SPMV_CH_CASE(10, 2)
     default: PM_SPMV_CASE(5, 32, 4)
   }
 #undef PM_SPMV_CASE
 #undef PM_SPMV_PF_CASE
+#undef PM_SPMV_CH_CASE
   return g;
 }
 

@@ -22,7 +22,7 @@ from .problem import NDF, NEN, Deck
 
 SOLVERS = {"pcg": 1, "gmres": 2, "bicgstab": 4, "minres": 7, "sqmr": 11}
 SOLVER_NAMES = {v: k for k, v in SOLVERS.items()}
-SPMV_SHAPES = 9  # PM_SPMV_SHAPES: (lanes per node block, columns in flight per lane) choices
+SPMV_SHAPES = 11  # PM_SPMV_SHAPES: (lanes per node block, columns in flight per lane) choices
 
 
 @dataclass
@@ -244,7 +244,10 @@ class MefSolver:
         neq, nb, nnz, ncols = C.c_int(), C.c_int(), C.c_int64(), C.c_int64()
         L.check(self.lib.pm_matrix_info(self.ctx, C.byref(neq), C.byref(nb), C.byref(nnz),
                                         C.byref(ncols)), "pm_matrix_info")
-        return dict(neq=neq.value, nblocks=nb.value, nnz=int(nnz.value), ncols=int(ncols.value))
+        nruns = C.c_int64()
+        L.check(self.lib.pm_matrix_runs(self.ctx, C.byref(nruns)), "pm_matrix_runs")
+        return dict(neq=neq.value, nblocks=nb.value, nnz=int(nnz.value), ncols=int(ncols.value),
+                    nruns=int(nruns.value))
 
     def bench_spmv(self, use32: bool, reps: int = 20) -> float:
         ms = C.c_double()
