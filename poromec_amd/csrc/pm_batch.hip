@@ -114,6 +114,75 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, int neq, const in
   }
 }
 
+// The batched product with one run of <= 4 consecutive columns per lane, as
+// k_spmv_ch: each workgroup streams its part of its box's node blocks.
+template <typename VT, int UNR, int NV>
+__global__ __launch_bounds__(PM_WG) void k_spmv_b_ch(
+    BatchArgs B, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
+    const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
+    const int64_t *__restrict__ chptr, const unsigned int *__restrict__ chunk,
+    const VT *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
+    const double *__restrict__ w, const PmScalars *S) {
+  const int box = B.wg_box[blockIdx.x];
+  if (S[box].done) return;
+  constexpr int GROUPS = PM_WG / 64;
+  const __amdgpu_buffer_rsrc_t xr = pm_rsrc(x, (unsigned int)neq * 8u);
+  const int lane = threadIdx.x % 64;
+  const int grp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+  const int gb = B.wg0[box], gc = B.wg0[box + 1] - gb;
+  const int64_t nb0 = B.blk0[box], nblk = B.blk0[box + 1] - nb0;
+  int64_t c0, c1;
+  pm_chunk(nblk, blockIdx.x - gb, gc, c0, c1);
+  const int64_t k0 = nb0 + c0, k1 = nb0 + c1;
+  double d0 = 0.0, d1 = 0.0;
+  PmRunMeta cur = pm_run_meta(k0 + grp, k1, colptr, valptr, nr, chptr);
+  PmRunMeta nx = pm_run_meta(k0 + grp + GROUPS, k1, colptr, valptr, nr, chptr);
+  unsigned int dsc[UNR];
+  {
+    const __amdgpu_buffer_rsrc_t h0 = pm_rsrc(chunk + cur.hp, (unsigned int)cur.nh * 4u);
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      dsc[u] = __builtin_amdgcn_raw_buffer_load_b32(h0, (unsigned int)(lane + u * 64) * 4u, 0, 2);
+  }
+  for (int64_t k = k0 + grp; k < k1; k += GROUPS) {
+    const PmRunMeta nn = pm_run_meta(k + 2 * GROUPS, k1, colptr, valptr, nr, chptr);
+    const int n = cur.n;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    const __amdgpu_buffer_rsrc_t vr = pm_rsrc(
+        val + cur.vp, ((unsigned int)n * (unsigned int)cur.c + 3u) * (unsigned int)sizeof(VT));
+    const unsigned int rowb = (unsigned int)cur.c * (unsigned int)sizeof(VT);
+    const __amdgpu_buffer_rsrc_t hr = pm_rsrc(chunk + cur.hp, (unsigned int)cur.nh * 4u);
+    const __amdgpu_buffer_rsrc_t hrn = pm_rsrc(chunk + nx.hp, (unsigned int)nx.nh * 4u);
+    unsigned int dn[UNR];
+    unsigned int base = pm_run_pass<VT, UNR, true>(lane, 0, cur, 0u, dsc, vr, rowb, xr, hrn, dn, a);
+    for (int h0 = UNR * 64; h0 < cur.nh; h0 += UNR * 64) {
+      unsigned int dp[UNR], unused[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++)
+        dp[u] = __builtin_amdgcn_raw_buffer_load_b32(hr, (unsigned int)(h0 + lane + u * 64) * 4u, 0, 2);
+      base = pm_run_pass<VT, UNR, false>(lane, h0, cur, base, dp, vr, rowb, xr, hrn, unused, a);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) dsc[u] = dn[u];
+#pragma unroll
+    for (int q = 0; q < 4; q++) a[q] = pm_group_sum<64>(a[q]);
+    if (lane == 63) {
+      const int f = fr[k];
+      for (int q = 0; q < n; q++) {
+        y[f + q] = a[q];
+        if (NV >= 1) d0 += a[q] * w[f + q];
+        if (NV >= 2) d1 += a[q] * a[q];
+      }
+    }
+    cur = nx;
+    nx = nn;
+  }
+  if (NV > 0) {
+    double v[2] = {d0, d1};
+    pm_block_partials<NV>(v, B.partials);
+  }
+}
+
 template <int NV, class Body>
 __global__ __launch_bounds__(PM_WG) void k_vec_b(BatchArgs B, const PmScalars *S, Body body,
                                                  int ignore_done) {
@@ -188,7 +257,13 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
                        y, w,                                                                    \
                        B->d_S);                                                                 \
     break;
-  switch (cfg) {  // the batched kernel has neither the prefetch nor the run form: 6-10 run as 2-4
+#define PM_BSPMV_CH_CASE(id, U)                                                                 \
+  case id:                                                                                      \
+    hipLaunchKernelGGL((k_spmv_b_ch<VT, U, NV>), dim3(B->G), dim3(PM_WG), 0, c->stream,        \
+                       batch_args(B), A.neq, A.fr, A.nr, A.colptr, A.valptr, A.chptr, A.chunk,  \
+                       val, x, y, w, B->d_S);                                                   \
+    break;
+  switch (cfg) {  // the batched kernel has no column-prefetch form: 6-8 run as 2-4
     PM_BSPMV_CASE(0, 64, 1)
     PM_BSPMV_CASE(1, 64, 2)
     PM_BSPMV_CASE(2, 64, 3)
@@ -197,11 +272,12 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
     PM_BSPMV_CASE(6, 64, 3)
     PM_BSPMV_CASE(7, 64, 4)
     PM_BSPMV_CASE(8, 64, 5)
-    PM_BSPMV_CASE(9, 64, 3)
-    PM_BSPMV_CASE(10, 64, 4)
+    PM_BSPMV_CH_CASE(9, 1)
+    PM_BSPMV_CH_CASE(10, 2)
     default: PM_BSPMV_CASE(5, 32, 4)
   }
 #undef PM_BSPMV_CASE
+#undef PM_BSPMV_CH_CASE
 }
 
 template <int NV, class Epi>

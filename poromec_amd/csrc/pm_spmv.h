@@ -271,3 +271,94 @@ __device__ __forceinline__ void pm_spmv_block(int lane, const PmBlockMeta &m,
 #pragma unroll
   for (int q = 0; q < 4; q++) a[q] = pm_group_sum<LANES>(a[q]);
 }
+
+// Runs of <= 4 consecutive columns, one per lane (k_spmv_ch, k_spmv_b_ch):
+// a block's metadata, and one pass of UNR runs per lane.
+struct PmRunMeta {
+  int64_t vp, hp;
+  int c, n, nh;
+};
+
+__device__ __forceinline__ PmRunMeta pm_run_meta(int64_t k, int64_t k1,
+                                                 const int64_t *__restrict__ colptr,
+                                                 const int64_t *__restrict__ valptr,
+                                                 const int *__restrict__ nr,
+                                                 const int64_t *__restrict__ chptr) {
+  PmRunMeta m{0, 0, 0, 0, 0};
+  if (k < k1) {
+    m.vp = valptr[k];
+    m.c = (int)(colptr[k + 1] - colptr[k]);
+    m.n = nr[k];
+    m.hp = chptr[k];
+    m.nh = (int)(chptr[k + 1] - m.hp);
+  }
+  return m;
+}
+
+__device__ __forceinline__ unsigned int pm_lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned int)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
+}
+
+// one pass of UNR runs per lane over block m: d[] the descriptors, base the
+// block columns before the pass; returns the columns the pass covered
+template <typename VT, int UNR, bool PREFETCH>
+__device__ __forceinline__ unsigned int pm_run_pass(int lane, int h0, const PmRunMeta &m,
+                                                    unsigned int base, const unsigned int (&d)[UNR],
+                                                    __amdgpu_buffer_rsrc_t vr, unsigned int rowb,
+                                                    __amdgpu_buffer_rsrc_t xr,
+                                                    __amdgpu_buffer_rsrc_t hrn,
+                                                    unsigned int (&dn)[UNR], double a[4]) {
+  constexpr int NT = 2;
+  constexpr int VPR = sizeof(VT) == 4 ? 1 : 2;  // 16-byte loads per row and run
+  unsigned int off[UNR];
+  int len[UNR];
+#pragma unroll
+  for (int u = 0; u < UNR; u++) {
+    const bool valid = h0 + lane + u * 64 < m.nh;
+    const unsigned int l1 = d[u] & 3u;
+    const unsigned long long bv = __ballot(valid);
+    const unsigned long long b0 = __ballot(valid && (l1 & 1u));
+    const unsigned long long b1 = __ballot(valid && (l1 & 2u));
+    off[u] = base + pm_lanes_below(bv) + pm_lanes_below(b0) + 2u * pm_lanes_below(b1);
+    len[u] = valid ? (int)l1 + 1 : 0;
+    base += (unsigned int)(__popcll(bv) + __popcll(b0) + 2 * __popcll(b1));
+  }
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  u4 w[UNR][4][VPR];
+#pragma unroll
+  for (int u = 0; u < UNR; u++) {
+    const unsigned int o = off[u] * (unsigned int)sizeof(VT);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int h = 0; h < VPR; h++)
+        w[u][q][h] = __builtin_bit_cast(
+            u4, __builtin_amdgcn_raw_buffer_load_b128(vr, o + (unsigned int)q * rowb + 16u * h, 0, NT));
+  }
+  double xv[UNR][4];
+#pragma unroll
+  for (int u = 0; u < UNR; u++) {
+    const unsigned int col = d[u] >> 2;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const double xc = pm_bload<double, 0>(xr, (col + (unsigned int)k) * 8u);
+      xv[u][k] = k < len[u] ? xc : 0.0;
+    }
+  }
+  if (PREFETCH) {
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      dn[u] = __builtin_amdgcn_raw_buffer_load_b32(hrn, (unsigned int)(lane + u * 64) * 4u, 0, NT);
+  }
+#pragma unroll
+  for (int u = 0; u < UNR; u++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      VT v[4];
+      __builtin_memcpy(v, &w[u][q][0], sizeof(v));
+#pragma unroll
+      for (int k = 0; k < 4; k++) a[q] += (double)v[k] * xv[u][k];
+    }
+  return base;
+}

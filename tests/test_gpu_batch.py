@@ -83,3 +83,23 @@ def test_batched_fp32ir():
     x32, i32 = s.solve(b, SolverConfig(tol=1e-10, precision="fp32ir"))
     assert i64["true_rel"] < 1e-9 and i32["true_rel"] < 1e-10
     assert np.abs(x32 - x64).max() <= 1e-8 * np.abs(x64).max()
+
+
+def test_batched_every_spmv_shape():
+    """Every SpMV shape of the batched kernel (the run-per-lane ones
+    included), FP64 and FP32+IR: each box reaches the tolerance and the
+    solution of the default shape."""
+    from poromec_amd.solver import SPMV_SHAPES, MefSolver, SolverConfig
+    decks, big, off = _boxes()
+    s = MefSolver(big)
+    s.batch_define(off)
+    s.pform(2, True, False, big.dt, big.dt)
+    b = np.random.default_rng(4).standard_normal(s.neq)
+    x0, _ = s.solve(b, SolverConfig(tol=1e-10))
+    for shape in range(SPMV_SHAPES):
+        s.set_spmv_shape(False, shape)
+        s.set_spmv_shape(True, shape)
+        for prec in ("fp64", "fp32ir"):
+            x, info = s.solve(b, SolverConfig(tol=1e-10, precision=prec))
+            assert info["true_rel"] < 1e-9, (shape, prec, info["true_rel"])
+            assert np.abs(x - x0).max() <= 1e-8 * np.abs(x0).max(), (shape, prec)
