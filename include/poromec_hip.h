@@ -259,8 +259,10 @@ int pm_matrix_runs(pm_ctx *ctx, int64_t *nruns);
  * stream. */
 int pm_bench_spmv(pm_ctx *ctx, int use32, int reps, double *ms_per_launch);
 /* SpMV shape tuner (no counterpart in the reference): times each of the
- * PM_SPMV_SHAPES (lanes per node block, columns in flight per lane) shapes
- * on the current matrix over `reps` launches, keeps the fastest for FP64
+ * PM_SPMV_SHAPES shapes on the current matrix over `reps` launches: 0-5
+ * (lanes per node block, columns in flight per lane), 6-8 the same with the
+ * column indices loaded one block ahead, 9-10 one or two runs of <= 4
+ * consecutive columns per lane.  It keeps the fastest for FP64
  * (use32 = 0) or FP32 values, and returns its index and, if ms_per_shape is
  * not NULL, every shape's average launch time.  The batching form tunes its
  * own kernel.  Shapes sum in different orders: results move at the rounding
