@@ -117,7 +117,7 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, int neq, const in
 // The batched product with one run of <= 4 consecutive columns per lane, as
 // k_spmv_ch: each workgroup streams its part of its box's node blocks.
 template <typename VT, int UNR, int NV>
-__global__ __launch_bounds__(PM_WG) void k_spmv_b_ch(
+__global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_b_ch(
     BatchArgs B, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
     const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
     const int64_t *__restrict__ chptr, const unsigned int *__restrict__ chunk,

@@ -149,8 +149,10 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_pf(
 // Loads past a run read the next columns' values (or the 4 values of padding
 // after the array) and are multiplied by zero; rows past the block's count
 // accumulate into sums that are not stored.
+// FP32 values: at least 6 waves per SIMD (two runs per lane then fit in 80
+// VGPRs without spills, against 101 and 4 waves unbounded)
 template <typename VT, int UNR, int NV, bool SCALE_M>
-__global__ __launch_bounds__(PM_WG) void k_spmv_ch(
+__global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_ch(
     int nrn, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
     const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
     const int64_t *__restrict__ chptr, const unsigned int *__restrict__ chunk,
