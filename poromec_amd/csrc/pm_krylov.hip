@@ -82,7 +82,7 @@ __global__ __launch_bounds__(PM_WG) void k_spmv(
 // block ahead (pm_spmv_block64_pf).  A wavefront's blocks k, k + GROUPS, ...
 // are wave-uniform (grp via readfirstlane), so their metadata loads are
 // scalar loads, two blocks ahead.
-template <typename VT, int UNR, int NV, bool SCALE_M>
+template <typename VT, int UNR, int NV, bool SCALE_M, bool SWEEP = false>
 __global__ __launch_bounds__(PM_WG) void k_spmv_pf(
     int nrn, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
     const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
@@ -95,11 +95,20 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_pf(
   const int lane = threadIdx.x % 64;
   const int grp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
   const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
-  int64_t k0, k1;
-  pm_chunk(nrn, lb, gridDim.x, k0, k1);
+  int64_t k0, k1, kfirst, kstep;
+  if (SWEEP && (gridDim.x & 7) == 0) {  // as in k_spmv_ch
+    const int W = (int)(gridDim.x >> 3);
+    pm_chunk(nrn, lb / W, 8, k0, k1);
+    kfirst = k0 + (int64_t)(lb % W) * GROUPS + grp;
+    kstep = (int64_t)W * GROUPS;
+  } else {
+    pm_chunk(nrn, lb, gridDim.x, k0, k1);
+    kfirst = k0 + grp;
+    kstep = GROUPS;
+  }
   double d0 = 0.0, d1 = 0.0;
-  PmBlockMeta cur = pm_uniform<64>(pm_block_meta(k0 + grp, k1, colptr, valptr, nr));
-  PmBlockMeta nx = pm_uniform<64>(pm_block_meta(k0 + grp + GROUPS, k1, colptr, valptr, nr));
+  PmBlockMeta cur = pm_uniform<64>(pm_block_meta(kfirst, k1, colptr, valptr, nr));
+  PmBlockMeta nx = pm_uniform<64>(pm_block_meta(kfirst + kstep, k1, colptr, valptr, nr));
   unsigned int col[UNR];
   {
     const __amdgpu_buffer_rsrc_t c0 = pm_rsrc(ncol + cur.cp, (unsigned int)cur.c * 4u);
@@ -107,8 +116,8 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_pf(
     for (int u = 0; u < UNR; u++)
       col[u] = __builtin_amdgcn_raw_buffer_load_b32(c0, (unsigned int)(lane + u * 64) * 4u, 0, 2);
   }
-  for (int64_t k = k0 + grp; k < k1; k += GROUPS) {
-    const PmBlockMeta nn = pm_uniform<64>(pm_block_meta(k + 2 * GROUPS, k1, colptr, valptr, nr));
+  for (int64_t k = kfirst; k < k1; k += kstep) {
+    const PmBlockMeta nn = pm_uniform<64>(pm_block_meta(k + 2 * kstep, k1, colptr, valptr, nr));
     const int n = cur.n;
     const int f = fr[k];
     double wq[4] = {0.0, 0.0, 0.0, 0.0}, mq[4] = {1.0, 1.0, 1.0, 1.0};
@@ -151,7 +160,11 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_pf(
 // accumulate into sums that are not stored.
 // FP32 values: at least 6 waves per SIMD (two runs per lane then fit in 80
 // VGPRs without spills, against 101 and 4 waves unbounded)
-template <typename VT, int UNR, int NV, bool SCALE_M>
+// SWEEP: the workgroups of an XCD sweep its eighth of the blocks together,
+// wave slot s taking blocks s, s + S, s + 2S, ... (S slots in the XCD), so
+// the x the XCD gathers at a time spans about the matrix band instead of its
+// whole eighth of x; otherwise each workgroup streams a contiguous chunk.
+template <typename VT, int UNR, int NV, bool SCALE_M, bool SWEEP = false>
 __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_ch(
     int nrn, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
     const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
@@ -165,11 +178,20 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_ch(
   const int lane = threadIdx.x % 64;
   const int grp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
   const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
-  int64_t k0, k1;
-  pm_chunk(nrn, lb, gridDim.x, k0, k1);
+  int64_t k0, k1, kfirst, kstep;
+  if (SWEEP && (gridDim.x & 7) == 0) {
+    const int W = (int)(gridDim.x >> 3);
+    pm_chunk(nrn, lb / W, 8, k0, k1);
+    kfirst = k0 + (int64_t)(lb % W) * GROUPS + grp;
+    kstep = (int64_t)W * GROUPS;
+  } else {
+    pm_chunk(nrn, lb, gridDim.x, k0, k1);
+    kfirst = k0 + grp;
+    kstep = GROUPS;
+  }
   double d0 = 0.0, d1 = 0.0;
-  PmRunMeta cur = pm_run_meta(k0 + grp, k1, colptr, valptr, nr, chptr);
-  PmRunMeta nx = pm_run_meta(k0 + grp + GROUPS, k1, colptr, valptr, nr, chptr);
+  PmRunMeta cur = pm_run_meta(kfirst, k1, colptr, valptr, nr, chptr);
+  PmRunMeta nx = pm_run_meta(kfirst + kstep, k1, colptr, valptr, nr, chptr);
   unsigned int dsc[UNR];
   {
     const __amdgpu_buffer_rsrc_t h0 = pm_rsrc(chunk + cur.hp, (unsigned int)cur.nh * 4u);
@@ -177,8 +199,8 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_ch(
     for (int u = 0; u < UNR; u++)
       dsc[u] = __builtin_amdgcn_raw_buffer_load_b32(h0, (unsigned int)(lane + u * 64) * 4u, 0, 2);
   }
-  for (int64_t k = k0 + grp; k < k1; k += GROUPS) {
-    const PmRunMeta nn = pm_run_meta(k + 2 * GROUPS, k1, colptr, valptr, nr, chptr);
+  for (int64_t k = kfirst; k < k1; k += kstep) {
+    const PmRunMeta nn = pm_run_meta(k + 2 * kstep, k1, colptr, valptr, nr, chptr);
     const int n = cur.n;
     const int f = fr[k];
     double wq[4] = {0.0, 0.0, 0.0, 0.0}, mq[4] = {1.0, 1.0, 1.0, 1.0};
@@ -265,15 +287,15 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
                        A.nrn, A.neq, A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w, m,   \
                        c->d_partials, S);                                                       \
     break;
-#define PM_SPMV_PF_CASE(id, U)                                                                  \
+#define PM_SPMV_PF_CASE(id, U, SW)                                                              \
   case id:                                                                                      \
-    hipLaunchKernelGGL((k_spmv_pf<VT, U, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream,    \
+    hipLaunchKernelGGL((k_spmv_pf<VT, U, NV, SCALE_M, SW>), dim3(g), dim3(PM_WG), 0, c->stream,\
                        A.nrn, A.neq, A.fr, A.nr, A.colptr, A.valptr, A.ncol, val, x, y, w, m,   \
                        c->d_partials, S);                                                       \
     break;
-#define PM_SPMV_CH_CASE(id, U)                                                                  \
+#define PM_SPMV_CH_CASE(id, U, SW)                                                              \
   case id:                                                                                      \
-    hipLaunchKernelGGL((k_spmv_ch<VT, U, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream,    \
+    hipLaunchKernelGGL((k_spmv_ch<VT, U, NV, SCALE_M, SW>), dim3(g), dim3(PM_WG), 0, c->stream,\
                        A.nrn, A.neq, A.fr, A.nr, A.colptr, A.valptr, A.chptr, A.chunk, val, x,  \
                        y, w, m, c->d_partials, S);                                              \
     break;
@@ -283,11 +305,14 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
     PM_SPMV_CASE(2, 64, 3)
     PM_SPMV_CASE(3, 64, 4)
     PM_SPMV_CASE(4, 64, 5)
-    PM_SPMV_PF_CASE(6, 3)
-    PM_SPMV_PF_CASE(7, 4)
-    PM_SPMV_PF_CASE(8, 5)
-    PM_SPMV_CH_CASE(9, 1)
-    PM_SPMV_CH_CASE(10, 2)
+    PM_SPMV_PF_CASE(6, 3, false)
+    PM_SPMV_PF_CASE(7, 4, false)
+    PM_SPMV_PF_CASE(8, 5, false)
+    PM_SPMV_PF_CASE(13, 5, true)
+    PM_SPMV_CH_CASE(9, 1, false)
+    PM_SPMV_CH_CASE(10, 2, false)
+    PM_SPMV_CH_CASE(11, 1, true)
+    PM_SPMV_CH_CASE(12, 2, true)
     default: PM_SPMV_CASE(5, 32, 4)
   }
 #undef PM_SPMV_CASE

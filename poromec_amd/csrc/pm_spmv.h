@@ -15,11 +15,12 @@
 // (LANES, UNR, PF) of each SpMV shape the tuner may pick; PF 1: the first
 // pass's column indices are loaded one block ahead (pm_spmv_block64_pf);
 // PF 2: one run of <= 4 consecutive columns per lane, UNR runs in flight
-// (k_spmv_ch)
-#define PM_SPMV_NCFG 11
-static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 64, 64, 32, 64, 64, 64, 64, 64};
-static constexpr int pm_spmv_unr[PM_SPMV_NCFG] = {1, 2, 3, 4, 5, 4, 3, 4, 5, 1, 2};
-static constexpr int pm_spmv_pf[PM_SPMV_NCFG] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2};
+// (k_spmv_ch); PF 3: the same with the XCD's workgroups sweeping its blocks
+// together; PF 4: PF 1 sweeping
+#define PM_SPMV_NCFG 14
+static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 64, 64, 32, 64, 64, 64, 64, 64, 64, 64, 64};
+static constexpr int pm_spmv_unr[PM_SPMV_NCFG] = {1, 2, 3, 4, 5, 4, 3, 4, 5, 1, 2, 1, 2, 5};
+static constexpr int pm_spmv_pf[PM_SPMV_NCFG] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 3, 3, 4};
 
 struct PmBlockMeta {
   int64_t cp, vp;
