@@ -10,6 +10,9 @@
 // a converged box return at entry, so finished boxes cost nothing.  Chunks of
 // iterations are captured once in a hipGraph and replayed.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -345,8 +348,10 @@ static void bchunk(pm_ctx *c, int solver, int use32, int chunk, const double *m,
 }
 
 // batched solve of every box: b, x, m device vectors of neq
+// cap_stragglers != 0: once half the boxes have converged at iteration L,
+// stop at 4 L; the boxes still open are left unconverged (PM_ERR_MAXIT)
 int pm_batch_krylov(pm_ctx *c, int solver, int maxit, double tol, const double *b, double *x,
-                    const double *m, int use32, pm_solve_info *info) {
+                    const double *m, int use32, pm_solve_info *info, int cap_stragglers) {
   PmBatch *B = c->batch;
   if (solver != PM_SOLVER_SQMR && solver != PM_SOLVER_PCG) return PM_ERR_UNSUPPORTED;
   PM_TRY(breset(c));
@@ -376,13 +381,31 @@ int pm_batch_krylov(pm_ctx *c, int solver, int maxit, double tol, const double *
     (void)hipGraphDestroy(g);
     memcpy(B->graph_key, key, sizeof(key));
   }
+  // PM_PROGRESS_SECONDS=s (opt-in): report the finished boxes on stderr every s seconds
+  static const double progress_s = [] {
+    const char *e = std::getenv("PM_PROGRESS_SECONDS");
+    return e ? std::atof(e) : 0.0;
+  }();
+  auto last = std::chrono::steady_clock::now();
+  int half_at = -1;
   while (ndone < B->nbox && launched < maxit) {
+    if (cap_stragglers && half_at >= 0 && launched >= 4 * std::max(half_at, 64)) break;
     if (B->use_graph) PM_CHECK_HIP(hipGraphLaunch(B->graph, c->stream));
     else bchunk(c, solver, use32, chunk, m, x);
     launched += chunk;
     (use32 ? info->spmv32 : info->spmv64) += chunk;
     PM_CHECK_HIP(hipGetLastError());
     PM_TRY(bpoll(c, &ndone));
+    if (half_at < 0 && 2 * ndone >= B->nbox) half_at = launched;
+    if (progress_s > 0.0) {
+      const auto now = std::chrono::steady_clock::now();
+      if (std::chrono::duration<double>(now - last).count() >= progress_s) {
+        std::fprintf(stderr, "[pm_batch] %s: %d of %d boxes done after %d iterations\n",
+                     use32 ? "fp32" : "fp64", ndone, B->nbox, launched);
+        std::fflush(stderr);
+        last = now;
+      }
+    }
   }
   PM_CHECK_HIP(hipMemcpyAsync(B->h_S, B->d_S, sizeof(PmScalars) * B->nbox, hipMemcpyDeviceToHost,
                               c->stream));
@@ -437,6 +460,13 @@ int pm_batch_resnorms(pm_ctx *c, const double *b, std::vector<double> &ru2,
     ru2[k] = B->h_S[k].red[0];
     rp2[k] = B->h_S[k].red[1];
   }
+  return PM_OK;
+}
+
+int pm_batch_unconverged(pm_ctx *c, std::vector<int> &open) {
+  PmBatch *B = c->batch;
+  open.assign(B->nbox, 0);
+  for (int k = 0; k < B->nbox; k++) open[k] = B->h_S[k].done == 0;
   return PM_OK;
 }
 

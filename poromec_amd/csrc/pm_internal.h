@@ -235,10 +235,13 @@ int pm_batch_nbox(pm_ctx *c);
 int pm_batch_last_iters(pm_ctx *c, int *iters);
 void pm_batch_set_graph(pm_ctx *c, int on);
 int pm_batch_krylov(pm_ctx *c, int solver, int maxit, double tol, const double *b, double *x,
-                    const double *m, int use32, pm_solve_info *info);
+                    const double *m, int use32, pm_solve_info *info,
+                    int cap_stragglers = 0);
 int pm_batch_resnorms(pm_ctx *c, const double *b, std::vector<double> &ru2,
                       std::vector<double> &rp2);
 int pm_batch_mask_rhs(pm_ctx *c, double *b, const std::vector<int> &active);
+// boxes whose last batched solve ended unconverged (1) or converged (0)
+int pm_batch_unconverged(pm_ctx *c, std::vector<int> &open);
 int pm_batch_true_res(pm_ctx *c, const double *b, const double *x, double *res,
                       std::vector<double> &r2, std::vector<double> &b2);
 int pm_adjoint_dev(pm_ctx *c, int nsteps, double dt, const pm_solver_cfg *cfg, int nsens,

@@ -1254,10 +1254,30 @@ int pm_solve_dev(pm_ctx *c, const pm_solver_cfg *cfg, const double *b, double *x
       for (int o = 0; o < maxouter && !all; o++) {
         pm_solve_info sub;
         memset(&sub, 0, sizeof(sub));
-        PM_TRY(pm_batch_krylov(c, cfg->solver, cfg->maxit, inner, rr, dx, m32, 1, &sub));
+        const int rc32 = pm_batch_krylov(c, cfg->solver, cfg->maxit, inner, rr, dx, m32, 1, &sub,
+                                         /*cap_stragglers=*/1);
+        if (rc32 != PM_OK && rc32 != PM_ERR_MAXIT) return rc32;
         info->iters += sub.iters;
         info->spmv32 += sub.spmv32;
         info->outer++;
+        if (rc32 == PM_ERR_MAXIT) {
+          // boxes whose FP32 inner solve did not converge: their correction
+          // is solved again with FP64 values, the other boxes' kept
+          std::vector<int> open, keep(nbox);
+          PM_TRY(pm_batch_unconverged(c, open));
+          for (int k = 0; k < nbox; k++) keep[k] = !open[k];
+          double *rr64 = c->d_wk[10], *dx64 = c->d_wk[11];
+          launch_vec<0>(c, n, c->d_scal, CopyBody{rr64, rr}, NoEpi{}, 1);
+          PM_TRY(pm_batch_mask_rhs(c, rr64, open));
+          PM_CHECK_HIP(hipMemsetAsync(dx64, 0, sizeof(double) * n, c->stream));
+          pm_solve_info sub64;
+          memset(&sub64, 0, sizeof(sub64));
+          PM_TRY(pm_batch_krylov(c, cfg->solver, cfg->maxit, inner, rr64, dx64, c->d_m, 0, &sub64));
+          info->iters += sub64.iters;
+          info->spmv64 += sub64.spmv64;
+          PM_TRY(pm_batch_mask_rhs(c, dx, keep));
+          launch_vec<0>(c, n, c->d_scal, AxpyBody{dx, dx64}, NoEpi{}, 1);
+        }
         launch_vec<0>(c, n, c->d_scal, AxpyBody{x, dx}, NoEpi{}, 1);
         PM_TRY(pm_batch_true_res(c, b, x, res, r2, b2));
         info->spmv64++;

@@ -103,3 +103,23 @@ def test_batched_every_spmv_shape():
             x, info = s.solve(b, SolverConfig(tol=1e-10, precision=prec))
             assert info["true_rel"] < 1e-9, (shape, prec, info["true_rel"])
             assert np.abs(x - x0).max() <= 1e-8 * np.abs(x0).max(), (shape, prec)
+
+
+def test_batched_fp32ir_straggler_fallback():
+    """One box far larger than the others: once half the boxes have converged
+    at L inner iterations, the FP32 inner solve stops at 4 L and the boxes
+    still open get their correction from an FP64 solve.  The result still
+    reaches the tolerance and the FP64 solution."""
+    from poromec_amd.solver import MefSolver, SolverConfig
+    decks = [P.heterogeneous_box_deck(n, seed=30 + n + k, sigma_lnk=s, dt=0.1, corr_len=2.0)
+             for k, (n, s) in enumerate(((2, 0.3), (2, 0.3), (2, 0.3), (7, 1.5)))]
+    big, off = P.concat_decks(decks)
+    s = MefSolver(big)
+    s.batch_define(off)
+    s.pform(2, True, False, big.dt, big.dt)
+    b = np.random.default_rng(5).standard_normal(s.neq)
+    x64, _ = s.solve(b, SolverConfig(tol=1e-10))
+    x32, info = s.solve(b, SolverConfig(tol=1e-10, precision="fp32ir"))
+    assert info["spmv64"] > info["outer"], info  # FP64 fallback products beyond the residuals
+    assert info["true_rel"] < 1e-10
+    assert np.abs(x32 - x64).max() <= 1e-8 * np.abs(x64).max()
