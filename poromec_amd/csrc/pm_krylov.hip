@@ -1337,7 +1337,15 @@ int pm_solve_dev(pm_ctx *c, const pm_solver_cfg *cfg, const double *b, double *x
       memset(&sub, 0, sizeof(sub));
       // stop the inner solve once it meets the outer target as well
       double itol = std::max(inner, 0.5 * cfg->tol / std::max(rel, 1e-300));
-      PM_TRY(krylov(c, cfg, n, rr, dx, m32, std::min(itol, 0.5), 1, &sub));
+      int rc32 = krylov(c, cfg, n, rr, dx, m32, std::min(itol, 0.5), 1, &sub);
+      if (rc32 == PM_ERR_MAXIT || rc32 == PM_ERR_BREAKDOWN) {
+        // the FP32 inner solve failed: this correction is solved with FP64 values
+        info->iters += sub.iters;
+        info->spmv32 += sub.spmv32;
+        memset(&sub, 0, sizeof(sub));
+        rc32 = krylov(c, cfg, n, rr, dx, c->d_m, std::min(itol, 0.5), 0, &sub);
+      }
+      PM_TRY(rc32);
       info->iters += sub.iters;
       info->spmv32 += sub.spmv32;
       info->spmv64 += sub.spmv64;
