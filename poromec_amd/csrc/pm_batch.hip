@@ -34,6 +34,7 @@ struct PmBatch {
   hipGraphExec_t graph = nullptr;
   uintptr_t graph_key[7] = {0, 0, 0, 0, 0, 0};
   int use_graph = 1;
+  int rebalance = 1;  // hand converged boxes' workgroups to the open ones
   ~PmBatch() {
     if (graph) (void)hipGraphExecDestroy(graph);
     (void)hipFree(d_wg_box);
@@ -327,6 +328,48 @@ static int bpoll(pm_ctx *c, int *ndone) {
   return PM_OK;
 }
 
+// Hands the launch's G workgroups to the boxes in proportion to their rows:
+// all boxes (open == nullptr, the definition's assignment) or the open ones
+// only (at least one each, the remainder to the largest), so boxes that are
+// still iterating get the workgroups of the converged ones.  The kernels
+// read the assignment from device memory at every launch, captured graphs
+// included.
+static int bassign(pm_ctx *c, const std::vector<int> *open) {
+  PmBatch *B = c->batch;
+  const int nbox = B->nbox, G = B->G;
+  std::vector<int> wg0(nbox + 1, 0);
+  if (open == nullptr) {
+    wg0 = B->wg0;
+  } else {
+    int64_t tot = 0;
+    int nopen = 0, big = -1;
+    for (int b = 0; b < nbox; b++)
+      if ((*open)[b]) {
+        const int rows = B->eq0[b + 1] - B->eq0[b];
+        tot += rows;
+        nopen++;
+        if (big < 0 || rows > B->eq0[big + 1] - B->eq0[big]) big = b;
+      }
+    if (nopen == 0 || nopen > G) return PM_OK;
+    std::vector<int> a(nbox, 0);
+    int used = 0;
+    for (int b = 0; b < nbox; b++)
+      if ((*open)[b]) {
+        a[b] = 1 + (int)((double)(B->eq0[b + 1] - B->eq0[b]) / (double)tot * (G - nopen));
+        used += a[b];
+      }
+    a[big] += G - used;
+    for (int b = 0; b < nbox; b++) wg0[b + 1] = wg0[b] + a[b];
+  }
+  std::vector<int> wg_box(G);
+  for (int b = 0; b < nbox; b++)
+    for (int g = wg0[b]; g < wg0[b + 1]; g++) wg_box[g] = b;
+  PM_CHECK_HIP(hipStreamSynchronize(c->stream));
+  PM_CHECK_HIP(hipMemcpy(B->d_wg0, wg0.data(), sizeof(int) * (nbox + 1), hipMemcpyHostToDevice));
+  PM_CHECK_HIP(hipMemcpy(B->d_wg_box, wg_box.data(), sizeof(int) * G, hipMemcpyHostToDevice));
+  return PM_OK;
+}
+
 // one chunk of iterations of the batched solver
 static void bchunk(pm_ctx *c, int solver, int use32, int chunk, const double *m, double *x) {
   PmBatch *B = c->batch;
@@ -388,6 +431,10 @@ int pm_batch_krylov(pm_ctx *c, int solver, int maxit, double tol, const double *
   }();
   auto last = std::chrono::steady_clock::now();
   int half_at = -1;
+  // each time the open boxes drop to 3/4 of the last count, their
+  // workgroups are handed out again among the open ones
+  int open_last = B->nbox - ndone;
+  bool reassigned = false;
   while (ndone < B->nbox && launched < maxit) {
     if (cap_stragglers && half_at >= 0 && launched >= 4 * std::max(half_at, 64)) break;
     if (B->use_graph) PM_CHECK_HIP(hipGraphLaunch(B->graph, c->stream));
@@ -397,6 +444,15 @@ int pm_batch_krylov(pm_ctx *c, int solver, int maxit, double tol, const double *
     PM_CHECK_HIP(hipGetLastError());
     PM_TRY(bpoll(c, &ndone));
     if (half_at < 0 && 2 * ndone >= B->nbox) half_at = launched;
+    const int open_now = B->nbox - ndone;
+    if (B->rebalance && open_now > 0 && 4 * open_now <= 3 * open_last) {
+      PM_CHECK_HIP(hipMemcpy(B->h_S, B->d_S, sizeof(PmScalars) * B->nbox, hipMemcpyDeviceToHost));
+      std::vector<int> open(B->nbox);
+      for (int k = 0; k < B->nbox; k++) open[k] = B->h_S[k].done == 0;
+      PM_TRY(bassign(c, &open));
+      reassigned = true;
+      open_last = open_now;
+    }
     if (progress_s > 0.0) {
       const auto now = std::chrono::steady_clock::now();
       if (std::chrono::duration<double>(now - last).count() >= progress_s) {
@@ -407,6 +463,7 @@ int pm_batch_krylov(pm_ctx *c, int solver, int maxit, double tol, const double *
       }
     }
   }
+  if (reassigned) PM_TRY(bassign(c, nullptr));  // the vector passes after a solve cover every box
   PM_CHECK_HIP(hipMemcpyAsync(B->h_S, B->d_S, sizeof(PmScalars) * B->nbox, hipMemcpyDeviceToHost,
                               c->stream));
   PM_CHECK_HIP(hipStreamSynchronize(c->stream));
