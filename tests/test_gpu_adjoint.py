@@ -3,7 +3,9 @@
 The gradient of a sensor misfit over a march with respect to per-material
 log E and log permeability must match central finite differences of the
 same device march (relative error 1e-5 along random and coordinate
-directions, Krylov tolerance 1e-13 so that each step is one exact solve).
+directions, Krylov tolerance 1e-13 so that each step is one exact solve), and
+central finite differences of the oracle's own march (the FP64 reference
+gradient).
 Checkpoint spacing must not change J, the gradients or the final state, and
 the torch.autograd wrapper must hand back the library's gradient."""
 import numpy as np
@@ -136,3 +138,40 @@ def test_adjoint_rejects_bad_sensors():
     with pytest.raises(PmError):
         s.adjoint(2, [(s.nnode, 0)], np.zeros((2, 1)), _cfg())
     s.close()
+
+
+def test_adjoint_matches_oracle_finite_differences():
+    """The device adjoint gradient against central differences of the
+    oracle's own forward march (the CPU restatement of the reference's
+    step): the misfit the oracle accumulates over the same sensors and
+    targets, along two seeded directions in (log E, log k)."""
+    import copy
+
+    from oracle import oracle as O
+    m = _misfit(False)
+    lE, lk = (t.numpy() for t in m.params())
+    J, gE, gk = m._evaluate(lE, lk)
+
+    def j_oracle(le, lkk):
+        d = copy.deepcopy(m.deck)
+        d.e = m.e0.copy()
+        d.e[:, 0] = np.exp(le)
+        d.e[:, 2] = np.exp(lkk)
+        om = O.OracleModel(d, solvtol=1e-13, maxit=200000)
+        total = 0.0
+        for n in range(NSTEPS):
+            om.step()
+            total += sum((om.u[a, b] - m.target[n, i]) ** 2 for i, (a, b) in enumerate(m.sensors))
+        return total
+
+    assert abs(j_oracle(lE, lk) - J) <= 1e-8 * J
+    rng = np.random.default_rng(21)
+    h = 1e-4
+    g = np.concatenate([gE, gk])
+    for _ in range(2):
+        v = rng.standard_normal(2 * len(lE))
+        vE, vk = v[: len(lE)], v[len(lE):]
+        fd = (j_oracle(lE + h * vE, lk + h * vk) - j_oracle(lE - h * vE, lk - h * vk)) / (2 * h)
+        ad = float(g @ v)
+        assert abs(fd - ad) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(g).sum()), (fd, ad)
+    m.close()
