@@ -32,15 +32,17 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "time-steps/s at 8.6M-dof Biot u-p (FP32+IR vs FP64 @1e-10); SpMV GB/s"
 # (lanes per node block, columns in flight per lane, index prefetch / runs)
-# of the SpMV shapes 0..13
-shape_lanes = (64, 64, 64, 64, 64, 32, 64, 64, 64, 64, 64, 64, 64, 64)
-shape_unr = (1, 2, 3, 4, 5, 4, 3, 4, 5, 1, 2, 1, 2, 5)
-shape_pf = (0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 3, 3, 4)
+# of the SpMV shapes 0..14
+shape_lanes = (64, 64, 64, 64, 64, 32, 64, 64, 64, 64, 64, 64, 64, 64, 64)
+shape_unr = (1, 2, 3, 4, 5, 4, 3, 4, 5, 1, 2, 1, 2, 5, 1)
+shape_pf = (0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 3, 3, 4, 5)
 
 
 def spmv_kernel_name(prec, shape):
     """The SpMV kernel the solver launches at this shape, with its fused dot."""
     vt = "float" if prec == "fp32" else "double"
+    if shape_pf[shape] == 5:
+        return f"k_spmv_cha<{vt}, 1, false>"
     if shape_pf[shape] == 4:
         return f"k_spmv_pf<{vt}, {shape_unr[shape]}, 1, false, true>"
     if shape_pf[shape] == 3:
@@ -65,7 +67,7 @@ def spmv_bytes(info, value_bytes, fused_dot=False, shape=None):
     each, first row and row count: 4 B each per block), or for the
     run-per-lane shapes one 32-bit word per run of <= 4 consecutive columns
     (plus chptr: 8 B per block)."""
-    runs = shape is not None and shape_pf[shape] in (2, 3)
+    runs = shape is not None and shape_pf[shape] in (2, 3, 5)
     cols = 4 * info["nruns"] if runs else 4 * info["ncols"]
     meta = (32 if runs else 24) * info["nblocks"]
     return value_bytes * info["nnz"] + cols + meta + (24 if fused_dot else 16) * info["neq"]

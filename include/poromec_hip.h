@@ -269,7 +269,7 @@ int pm_bench_spmv(pm_ctx *ctx, int use32, int reps, double *ms_per_launch);
  * not NULL, every shape's average launch time.  The batching form tunes its
  * own kernel.  Shapes sum in different orders: results move at the rounding
  * level when the shape changes, and stay fixed until the next change. */
-#define PM_SPMV_SHAPES 14
+#define PM_SPMV_SHAPES 15
 int pm_tune_spmv(pm_ctx *ctx, int use32, int reps, int *best, double *ms_per_shape);
 /* Selects SpMV shape `shape` (0 .. PM_SPMV_SHAPES-1) without timing. */
 int pm_set_spmv_shape(pm_ctx *ctx, int use32, int shape);

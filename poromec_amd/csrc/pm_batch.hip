@@ -267,7 +267,7 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
                        batch_args(B), A.neq, A.fr, A.nr, A.colptr, A.valptr, A.chptr, A.chunk,  \
                        val, x, y, w, B->d_S);                                                   \
     break;
-  switch (cfg) {  // no column-prefetch nor sweep form here: 6-8 and 13 run as 2-4, 11-12 as 9-10
+  switch (cfg) {  // no column-prefetch nor sweep form here: 6-8 and 13 run as 2-4, 11-12 and 14 as 9-10
     PM_BSPMV_CASE(0, 64, 1)
     PM_BSPMV_CASE(1, 64, 2)
     PM_BSPMV_CASE(2, 64, 3)
@@ -281,6 +281,7 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
     PM_BSPMV_CH_CASE(11, 1)
     PM_BSPMV_CH_CASE(12, 2)
     PM_BSPMV_CASE(13, 64, 5)
+    PM_BSPMV_CH_CASE(14, 1)
     default: PM_BSPMV_CASE(5, 32, 4)
   }
 #undef PM_BSPMV_CASE

@@ -247,6 +247,100 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_ch(
   }
 }
 
+// k_spmv_ch with the pass width chosen per block: blocks of <= 64 runs (the
+// mid-edge nodes') take one run per lane, wider blocks (the vertices', ~77
+// runs at 87^3) two, so neither needs a second round trip; two units of the
+// next block's descriptors are always loaded ahead.  XCD sweep mapping.
+template <typename VT, int NV, bool SCALE_M>
+__global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_cha(
+    int nrn, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
+    const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
+    const int64_t *__restrict__ chptr, const unsigned int *__restrict__ chunk,
+    const VT *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
+    const double *__restrict__ w, const double *__restrict__ m, double *partials,
+    const PmScalars *S) {
+  if (S != nullptr && S->done) return;
+  constexpr int GROUPS = PM_WG / 64;
+  const __amdgpu_buffer_rsrc_t xr = pm_rsrc(x, (unsigned int)neq * 8u);
+  const int lane = threadIdx.x % 64;
+  const int grp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+  const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
+  int64_t k0, k1, kfirst, kstep;
+  if ((gridDim.x & 7) == 0) {
+    const int W = (int)(gridDim.x >> 3);
+    pm_chunk(nrn, lb / W, 8, k0, k1);
+    kfirst = k0 + (int64_t)(lb % W) * GROUPS + grp;
+    kstep = (int64_t)W * GROUPS;
+  } else {
+    pm_chunk(nrn, lb, gridDim.x, k0, k1);
+    kfirst = k0 + grp;
+    kstep = GROUPS;
+  }
+  double d0 = 0.0, d1 = 0.0;
+  PmRunMeta cur = pm_run_meta(kfirst, k1, colptr, valptr, nr, chptr);
+  PmRunMeta nx = pm_run_meta(kfirst + kstep, k1, colptr, valptr, nr, chptr);
+  unsigned int dsc[2];
+  {
+    const __amdgpu_buffer_rsrc_t h0 = pm_rsrc(chunk + cur.hp, (unsigned int)cur.nh * 4u);
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+      dsc[u] = __builtin_amdgcn_raw_buffer_load_b32(h0, (unsigned int)(lane + u * 64) * 4u, 0, 2);
+  }
+  for (int64_t k = kfirst; k < k1; k += kstep) {
+    const PmRunMeta nn = pm_run_meta(k + 2 * kstep, k1, colptr, valptr, nr, chptr);
+    const int n = cur.n;
+    const int f = fr[k];
+    double wq[4] = {0.0, 0.0, 0.0, 0.0}, mq[4] = {1.0, 1.0, 1.0, 1.0};
+    if (lane == 63) {
+      for (int q = 0; q < n; q++) {
+        if (NV >= 1) wq[q] = w[f + q];
+        if (SCALE_M) mq[q] = m[f + q];
+      }
+    }
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    const __amdgpu_buffer_rsrc_t vr = pm_rsrc(
+        val + cur.vp, ((unsigned int)n * (unsigned int)cur.c + 3u) * (unsigned int)sizeof(VT));
+    const unsigned int rowb = (unsigned int)cur.c * (unsigned int)sizeof(VT);
+    const __amdgpu_buffer_rsrc_t hr = pm_rsrc(chunk + cur.hp, (unsigned int)cur.nh * 4u);
+    const __amdgpu_buffer_rsrc_t hrn = pm_rsrc(chunk + nx.hp, (unsigned int)nx.nh * 4u);
+    unsigned int dn[2];
+    unsigned int base;
+    if (cur.nh > 64) {
+      base = pm_run_pass<VT, 2, true, 2>(lane, 0, cur, 0u, dsc, vr, rowb, xr, hrn, dn, a);
+      for (int h0 = 128; h0 < cur.nh; h0 += 128) {
+        unsigned int dp[2], unused[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+          dp[u] = __builtin_amdgcn_raw_buffer_load_b32(hr, (unsigned int)(h0 + lane + u * 64) * 4u, 0, 2);
+        base = pm_run_pass<VT, 2, false>(lane, h0, cur, base, dp, vr, rowb, xr, hrn, unused, a);
+      }
+    } else {
+      const unsigned int d1[1] = {dsc[0]};
+      base = pm_run_pass<VT, 1, true, 2>(lane, 0, cur, 0u, d1, vr, rowb, xr, hrn, dn, a);
+    }
+    (void)base;
+    dsc[0] = dn[0];
+    dsc[1] = dn[1];
+#pragma unroll
+    for (int q = 0; q < 4; q++) a[q] = pm_group_sum<64>(a[q]);
+    if (lane == 63) {
+      for (int q = 0; q < n; q++) {
+        double yq = a[q];
+        if (SCALE_M) yq *= mq[q];
+        y[f + q] = yq;
+        if (NV >= 1) d0 += yq * wq[q];
+        if (NV >= 2) d1 += yq * yq;
+      }
+    }
+    cur = nx;
+    nx = nn;
+  }
+  if (NV > 0) {
+    double v[2] = {d0, d1};
+    pm_block_partials<NV>(v, partials);
+  }
+}
+
 // Sums the workgroup partials of the kernel before it (fixed order) into
 // S->red and runs the epilogue that computes the next solver scalars.
 template <int NV, class Epi>
@@ -309,6 +403,11 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
     PM_SPMV_PF_CASE(7, 4, false)
     PM_SPMV_PF_CASE(8, 5, false)
     PM_SPMV_PF_CASE(13, 5, true)
+    case 14:
+      hipLaunchKernelGGL((k_spmv_cha<VT, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream, A.nrn,
+                         A.neq, A.fr, A.nr, A.colptr, A.valptr, A.chptr, A.chunk, val, x, y, w, m,
+                         c->d_partials, S);
+      break;
     PM_SPMV_CH_CASE(9, 1, false)
     PM_SPMV_CH_CASE(10, 2, false)
     PM_SPMV_CH_CASE(11, 1, true)

@@ -16,11 +16,12 @@
 // pass's column indices are loaded one block ahead (pm_spmv_block64_pf);
 // PF 2: one run of <= 4 consecutive columns per lane, UNR runs in flight
 // (k_spmv_ch); PF 3: the same with the XCD's workgroups sweeping its blocks
-// together; PF 4: PF 1 sweeping
-#define PM_SPMV_NCFG 14
-static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 64, 64, 32, 64, 64, 64, 64, 64, 64, 64, 64};
-static constexpr int pm_spmv_unr[PM_SPMV_NCFG] = {1, 2, 3, 4, 5, 4, 3, 4, 5, 1, 2, 1, 2, 5};
-static constexpr int pm_spmv_pf[PM_SPMV_NCFG] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 3, 3, 4};
+// together; PF 4: PF 1 sweeping; PF 5: runs, one or two per lane by block
+// width, sweeping (k_spmv_cha)
+#define PM_SPMV_NCFG 15
+static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 64, 64, 32, 64, 64, 64, 64, 64, 64, 64, 64, 64};
+static constexpr int pm_spmv_unr[PM_SPMV_NCFG] = {1, 2, 3, 4, 5, 4, 3, 4, 5, 1, 2, 1, 2, 5, 1};
+static constexpr int pm_spmv_pf[PM_SPMV_NCFG] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 3, 3, 4, 5};
 
 struct PmBlockMeta {
   int64_t cp, vp;
@@ -303,13 +304,14 @@ __device__ __forceinline__ unsigned int pm_lanes_below(unsigned long long mask) 
 
 // one pass of UNR runs per lane over block m: d[] the descriptors, base the
 // block columns before the pass; returns the columns the pass covered
-template <typename VT, int UNR, bool PREFETCH>
+// (NPF: the units of the next block's descriptors loaded ahead, UNR by default)
+template <typename VT, int UNR, bool PREFETCH, int NPF = UNR>
 __device__ __forceinline__ unsigned int pm_run_pass(int lane, int h0, const PmRunMeta &m,
                                                     unsigned int base, const unsigned int (&d)[UNR],
                                                     __amdgpu_buffer_rsrc_t vr, unsigned int rowb,
                                                     __amdgpu_buffer_rsrc_t xr,
                                                     __amdgpu_buffer_rsrc_t hrn,
-                                                    unsigned int (&dn)[UNR], double a[4]) {
+                                                    unsigned int (&dn)[NPF], double a[4]) {
   constexpr int NT = 2;
   constexpr int VPR = sizeof(VT) == 4 ? 1 : 2;  // 16-byte loads per row and run
   unsigned int off[UNR];
@@ -349,7 +351,7 @@ __device__ __forceinline__ unsigned int pm_run_pass(int lane, int h0, const PmRu
   }
   if (PREFETCH) {
 #pragma unroll
-    for (int u = 0; u < UNR; u++)
+    for (int u = 0; u < NPF; u++)
       dn[u] = __builtin_amdgcn_raw_buffer_load_b32(hrn, (unsigned int)(lane + u * 64) * 4u, 0, NT);
   }
 #pragma unroll
