@@ -263,8 +263,9 @@ int pm_bench_spmv(pm_ctx *ctx, int use32, int reps, double *ms_per_launch);
  * (lanes per node block, columns in flight per lane), 6-8 the same with the
  * column indices loaded one block ahead, 9-10 one or two runs of <= 4
  * consecutive columns per lane, 11-12 the runs and 13 the (64, 5) prefetch
- * shape with the workgroups of each XCD sweeping its blocks together.  It
- * keeps the fastest for FP64
+ * shape with the workgroups of each XCD sweeping its blocks together, 14
+ * the sweeping runs with one run per lane in blocks of <= 64 runs and two in
+ * wider blocks.  It keeps the fastest for FP64
  * (use32 = 0) or FP32 values, and returns its index and, if ms_per_shape is
  * not NULL, every shape's average launch time.  The batching form tunes its
  * own kernel.  Shapes sum in different orders: results move at the rounding
