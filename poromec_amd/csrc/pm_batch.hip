@@ -120,7 +120,7 @@ __global__ __launch_bounds__(PM_WG) void k_spmv_b(BatchArgs B, int neq, const in
 
 // The batched product with one run of <= 4 consecutive columns per lane, as
 // k_spmv_ch: each workgroup streams its part of its box's node blocks.
-template <typename VT, int UNR, int NV>
+template <typename VT, int UNR, int NV, bool ADAPT = false>
 __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_b_ch(
     BatchArgs B, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
     const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_b_ch(
     for (int u = 0; u < UNR; u++)
       dsc[u] = __builtin_amdgcn_raw_buffer_load_b32(h0, (unsigned int)(lane + u * 64) * 4u, 0, 2);
   }
-  for (int64_t k = k0 + grp; k < k1; k += GROUPS) {
+  for (int64_t k = k0 + grp; k < k1; k += GROUPS) {  // ADAPT: blocks of <= 64 runs in one pass of one run per lane
     const PmRunMeta nn = pm_run_meta(k + 2 * GROUPS, k1, colptr, valptr, nr, chptr);
     const int n = cur.n;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
@@ -158,13 +158,18 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_b_ch(
     const __amdgpu_buffer_rsrc_t hr = pm_rsrc(chunk + cur.hp, (unsigned int)cur.nh * 4u);
     const __amdgpu_buffer_rsrc_t hrn = pm_rsrc(chunk + nx.hp, (unsigned int)nx.nh * 4u);
     unsigned int dn[UNR];
-    unsigned int base = pm_run_pass<VT, UNR, true>(lane, 0, cur, 0u, dsc, vr, rowb, xr, hrn, dn, a);
-    for (int h0 = UNR * 64; h0 < cur.nh; h0 += UNR * 64) {
-      unsigned int dp[UNR], unused[UNR];
+    if (ADAPT && cur.nh <= 64) {
+      const unsigned int d1[1] = {dsc[0]};
+      (void)pm_run_pass<VT, 1, true, UNR>(lane, 0, cur, 0u, d1, vr, rowb, xr, hrn, dn, a);
+    } else {
+      unsigned int base = pm_run_pass<VT, UNR, true>(lane, 0, cur, 0u, dsc, vr, rowb, xr, hrn, dn, a);
+      for (int h0 = UNR * 64; h0 < cur.nh; h0 += UNR * 64) {
+        unsigned int dp[UNR], unused[UNR];
 #pragma unroll
-      for (int u = 0; u < UNR; u++)
-        dp[u] = __builtin_amdgcn_raw_buffer_load_b32(hr, (unsigned int)(h0 + lane + u * 64) * 4u, 0, 2);
-      base = pm_run_pass<VT, UNR, false>(lane, h0, cur, base, dp, vr, rowb, xr, hrn, unused, a);
+        for (int u = 0; u < UNR; u++)
+          dp[u] = __builtin_amdgcn_raw_buffer_load_b32(hr, (unsigned int)(h0 + lane + u * 64) * 4u, 0, 2);
+        base = pm_run_pass<VT, UNR, false>(lane, h0, cur, base, dp, vr, rowb, xr, hrn, unused, a);
+      }
     }
 #pragma unroll
     for (int u = 0; u < UNR; u++) dsc[u] = dn[u];
@@ -267,7 +272,7 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
                        batch_args(B), A.neq, A.fr, A.nr, A.colptr, A.valptr, A.chptr, A.chunk,  \
                        val, x, y, w, B->d_S);                                                   \
     break;
-  switch (cfg) {  // no column-prefetch nor sweep form here: 6-8 and 13 run as 2-4, 11-12 and 14 as 9-10
+  switch (cfg) {  // no column-prefetch nor sweep form here: 6-8 and 13 run as 2-4, 11-12 as 9-10
     PM_BSPMV_CASE(0, 64, 1)
     PM_BSPMV_CASE(1, 64, 2)
     PM_BSPMV_CASE(2, 64, 3)
@@ -281,7 +286,11 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
     PM_BSPMV_CH_CASE(11, 1)
     PM_BSPMV_CH_CASE(12, 2)
     PM_BSPMV_CASE(13, 64, 5)
-    PM_BSPMV_CH_CASE(14, 1)
+    case 14:
+      hipLaunchKernelGGL((k_spmv_b_ch<VT, 2, NV, true>), dim3(B->G), dim3(PM_WG), 0, c->stream,
+                         batch_args(B), A.neq, A.fr, A.nr, A.colptr, A.valptr, A.chptr, A.chunk,
+                         val, x, y, w, B->d_S);
+      break;
     default: PM_BSPMV_CASE(5, 32, 4)
   }
 #undef PM_BSPMV_CASE
