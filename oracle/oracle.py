@@ -104,6 +104,20 @@ def _load_lib():
     L.orc_step.restype = C.c_int
     L.orc_step.argtypes = [C.POINTER(_Model), C.POINTER(_Cfg), C.POINTER(OracleEnv),
                            C.POINTER(_StepInfo)]
+    L.orc_solv.restype = C.c_int
+    L.orc_solv.argtypes = [C.c_int, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                           C.POINTER(_Cfg), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.omp_colour.restype = C.c_int
+    L.omp_colour.argtypes = [C.POINTER(_Model), _i32p, C.c_int]
+    L.omp_pform_sample.restype = C.c_double
+    L.omp_pform_sample.argtypes = [C.POINTER(_Model), C.c_int, C.c_int, C.c_int, _f64p,
+                                   C.POINTER(OracleEnv), _i32p, C.c_int, C.c_int64, C.c_int,
+                                   C.POINTER(C.c_int64)]
+    L.omp_rpsqmr.restype = C.c_int
+    L.omp_rpsqmr.argtypes = sig + [C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int)]
+    L.orc_residuo.restype = C.c_int
+    L.orc_residuo.argtypes = [C.POINTER(_Model), _f64p, C.c_int, C.c_double, _f64p, C.c_int]
     L.orc_initial_porosity.argtypes = [C.POINTER(_Model)]
     L.orc_initial_prop.restype = C.c_int
     L.orc_initial_prop.argtypes = [C.POINTER(_Model), _f64p, C.c_int]
@@ -282,15 +296,73 @@ class OracleModel:
             raise RuntimeError(f"orc_tform failed: {rc}")
         return tx, txe, txb, flux
 
-    def step(self):
+    def step(self, check=True):
+        """One time step (orc_step).  check=False returns the step's return
+        code in the result (0, or -10 when maxnlit is reached) instead of
+        raising on a nonzero one."""
         info = _StepInfo()
         rc = lib.orc_step(C.byref(self._md), C.byref(self.cfg), C.byref(self.env), C.byref(info))
-        if rc:
+        if rc and check:
             raise RuntimeError(f"orc_step failed: {rc}")
         self.t = self.env.t
         n = min(info.nsolve, 16)
         return dict(nlit=info.nlit, nsolve=info.nsolve, iters=list(info.iters[:n]),
-                    used=list(info.used[:n]), resid=list(info.resid[:n]))
+                    used=list(info.used[:n]), resid=list(info.resid[:n]), rc=rc)
+
+    def colour(self, maxcolors=200):
+        """Greedy element colouring (omp_colour): (ncolors, colour of each
+        element, 1-based)."""
+        col = np.zeros(self.st["numel"], np.int32)
+        nc = lib.omp_colour(C.byref(self._md), col, maxcolors)
+        return nc, col
+
+    def pform_colour(self, col, colour, isw=2, lhs=True, rhs=True, bvec=None, max_elems=None,
+                     nthreads=1):
+        """The OpenMP coloured element loop over (up to max_elems) elements of
+        one colour, adding into ad/al/bvec without zeroing them first.
+        Returns (wall seconds, elements done)."""
+        bvec = self.b if bvec is None else bvec
+        n = C.c_int64()
+        lim = self.st["numel"] if max_elems is None else max_elems
+        t = lib.omp_pform_sample(C.byref(self._md), isw, int(lhs), int(rhs), bvec,
+                                 C.byref(self.env), col, colour, lim, nthreads, C.byref(n))
+        return t, n.value
+
+    def residuo(self, b, nli, tol, state, crit):
+        """One cal_residuo_pm call (orc_residuo) on the vector b; state is the
+        float64[4] carried between the calls of one step.  Returns istop."""
+        return lib.orc_residuo(C.byref(self._md), np.ascontiguousarray(b, np.float64), nli, tol,
+                               state, crit)
+
+
+def omp_sqmr(st, ad, al, m, b, tol=1e-10, maxit=50000, nthreads=1):
+    """rpsqmr_omp (the reference's OpenMP SQMR) on nthreads threads.
+    Returns (return value, x, fail flag); the return value is the iteration
+    count, or -2 at maxit (fail = 1, the GMRES fallback's trigger)."""
+    neq = st["neq"]
+    x = np.zeros(neq)
+    res, fail = C.c_double(), C.c_int()
+    it = lib.omp_rpsqmr(neq, st["ia"], st["ja"], np.ascontiguousarray(ad, np.float64),
+                        np.ascontiguousarray(al, np.float64), np.ascontiguousarray(m, np.float64),
+                        np.ascontiguousarray(b, np.float64), x, tol, maxit, nthreads,
+                        C.byref(res), C.byref(fail))
+    return it, x, fail.value
+
+
+def solv(st, ad, al, b, solver=11, precond=2, tol=1e-10, maxit=50000, ngram=150):
+    """solv_pm's dispatch (orc_solv) on a CSRC system: the preconditioner, the
+    solver and, for SQMR, the GMRES fallback.  Returns (return value, x,
+    solver that finished)."""
+    neq = st["neq"]
+    m = np.zeros(neq)
+    x = np.zeros(neq)
+    cfg = _Cfg(solver, precond, maxit, ngram, 500, 1, 0, tol, 1e-4)
+    used, res = C.c_int(), C.c_double()
+    it = lib.orc_solv(neq, st["ia"], st["ja"], np.ascontiguousarray(ad, np.float64),
+                      np.ascontiguousarray(al, np.float64), m,
+                      np.ascontiguousarray(b, np.float64), x, C.byref(cfg), C.byref(used),
+                      C.byref(res))
+    return it, x, used.value
 
 
 # ---------------------------------------------------------------------------

@@ -33,7 +33,7 @@
 /* csr (Assbly.f:218-304): b(k) -= p(i); ad(k) += s(i,i); every column l of
  * row k is searched linearly in ja(ia(k):ia(k+1)-1) (columns l > k are not
  * stored and are dropped) */
-static void assemble_csr(const double *s, const double *p, const int *ld, const int64_t *ia,
+void orc_assemble_csr(const double *s, const double *p, const int *ld, const int64_t *ia,
                          const int *ja, double *al, double *ad, double *b, int neq, int lhs,
                          int rhs) {
   for (int i = 0; i < NST; i++) {
@@ -53,7 +53,7 @@ static void assemble_csr(const double *s, const double *p, const int *ld, const 
 }
 
 /* gathers the element arrays exactly as pform_pm does */
-static void gather(const orc_model *md, int nel, int *ld, double *ul, double *dpl, double *xl,
+void orc_gather(const orc_model *md, int nel, int *ld, double *ul, double *dpl, double *xl,
                    double *pl) {
   const int *ix = md->ix + (size_t)nel * NEN1;
   int kk = 0;
@@ -100,12 +100,12 @@ int orc_pform(orc_model *md, int isw, int lhs, int rhs, double *bvec, const orc_
   }
   int rc = 0;
   for (int nel = 0; nel < md->numel && rc == 0; nel++) {
-    gather(md, nel, ld, ul, dpl, xl, pl);
+    orc_gather(md, nel, ld, ul, dpl, xl, pl);
     int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
     rc = elmlib(md, nel, ma, md->eload + 7 * (size_t)nel, md->eloadp + 7 * (size_t)nel, xl, ul,
                 dpl, pl, sl, isw, env);
     if (rc) break;
-    assemble_csr(sl, pl, ld, md->ia, md->ja, md->al, md->ad, bvec, md->neq, lhs, rhs);
+    orc_assemble_csr(sl, pl, ld, md->ia, md->ja, md->al, md->ad, bvec, md->neq, lhs, rhs);
   }
   free(sl);
   return rc;
@@ -143,7 +143,7 @@ int orc_tform(orc_model *md, double *tx, double *txe, double *txb, double *flux,
   memset(flux, 0, sizeof(double) * 3 * (size_t)md->nnode);
   int rc = 0;
   for (int nel = 0; nel < md->numel; nel++) {
-    gather(md, nel, ld, ul, dpl, xl, pl);
+    orc_gather(md, nel, ld, ul, dpl, xl, pl);
     for (int i = 0; i < 8 * 15; i++) p[i] = 0.0;
     int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
     rc = elmlib(md, nel, ma, iqz, iqz, xl, ul, dpl, p, NULL, 3, env);
@@ -186,7 +186,7 @@ void orc_porosity(orc_model *md, double *dporo, int *ic, const orc_env *env) {
   int iqz[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int i = 0; i < md->nnode; i++) { ic[i] = 0; dporo[i] = 0.0; }
   for (int nel = 0; nel < md->numel; nel++) {
-    gather(md, nel, ld, ul, NULL, xl, pl);
+    orc_gather(md, nel, ld, ul, NULL, xl, pl);
     for (int i = 0; i < ORC_NENV; i++) pl[i] = 0.0;
     int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
     elmlib(md, nel, ma, iqz, iqz, xl, ul, NULL, pl, NULL, 7, env);
@@ -211,7 +211,7 @@ int orc_update_prop(orc_model *md, const orc_env *env) {
   double ul[NST], pl[NST], xl[3 * ORC_NENV], vpl[448];
   int iqz[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int nel = 0; nel < md->numel; nel++) {
-    gather(md, nel, ld, ul, NULL, xl, pl);
+    orc_gather(md, nel, ld, ul, NULL, xl, pl);
     int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
     if (elem_type(md, ma) != 18) continue;  /* elmt17_pm has no isw 10 */
     double *vp = md->vpropel + (size_t)nel * 448;

@@ -153,6 +153,23 @@ int orc_solv(int neq, const int64_t *ia, const int *ja, const double *ad, const 
              double *m, const double *b, double *x, const orc_cfg *cfg, int *used,
              double *res);
 
+/* element gather of pform_pm (Pform.f:307-381) and the csr scatter
+ * (Assbly.f:218-304) of one element */
+void orc_gather(const orc_model *md, int nel, int *ld, double *ul, double *dpl, double *xl,
+                double *pl);
+void orc_assemble_csr(const double *s, const double *p, const int *ld, const int64_t *ia,
+                      const int *ja, double *al, double *ad, double *b, int neq, int lhs,
+                      int rhs);
+
+/* ---- the reference's OpenMP CPU path (pm_omp.c), the timed CPU baseline ---- */
+int omp_colour(const orc_model *md, int *elcolor, int maxcolors);
+double omp_pform_sample(orc_model *md, int isw, int lhs, int rhs, double *bvec,
+                        const orc_env *env, const int *elcolor, int colour, int64_t max_elems,
+                        int nthreads, int64_t *nel_done);
+int omp_rpsqmr(int neq, const int64_t *ia, const int *ja, const double *ad, const double *al,
+               const double *m, const double *b, double *x, double tol, int maxit,
+               int nthreads, double *res, int *fail);
+
 /* ---- time step (Pload.f, Predict.f, Residuo.f, Main.f) ---- */
 void orc_pload(orc_model *md, const orc_env *env);
 void orc_delta_predict(orc_model *md);

@@ -32,7 +32,8 @@ struct PmBatch {
   std::vector<int> last_iters;
   // hipGraph cache of one chunk of iterations
   hipGraphExec_t graph = nullptr;
-  uintptr_t graph_key[7] = {0, 0, 0, 0, 0, 0};
+  static constexpr int kGraphKey = 20;
+  uintptr_t graph_key[kGraphKey] = {};
   int use_graph = 1;
   int rebalance = 1;  // hand converged boxes' workgroups to the open ones
   ~PmBatch() {
@@ -407,6 +408,8 @@ int pm_batch_krylov(pm_ctx *c, int solver, int maxit, double tol, const double *
                     const double *m, int use32, pm_solve_info *info, int cap_stragglers) {
   PmBatch *B = c->batch;
   if (solver != PM_SOLVER_SQMR && solver != PM_SOLVER_PCG) return PM_ERR_UNSUPPORTED;
+  // the box ranges must describe the matrix the kernels are about to read
+  if (B->eq0[B->nbox] != c->A.neq || B->blk0[B->nbox] != c->A.nrn) return PM_ERR_STATE;
   PM_TRY(breset(c));
   if (solver == PM_SOLVER_PCG) {
     double *r = c->d_wk[0], *z = c->d_wk[1], *p = c->d_wk[2];
@@ -419,10 +422,17 @@ int pm_batch_krylov(pm_ctx *c, int solver, int maxit, double tol, const double *
   int launched = 0, ndone = 0;
   PM_TRY(bpoll(c, &ndone));
   const int chunk = 16;
-  // graph of one chunk: kernels only, fixed pointers for this (solver, precision, x, m)
-  const uintptr_t key[7] = {(uintptr_t)solver, (uintptr_t)use32, (uintptr_t)x, (uintptr_t)m,
-                            (uintptr_t)c->d_wk[0], (uintptr_t)c->A.val32,
-                            (uintptr_t)c->spmv_cfg[use32 ? 1 : 0]};
+  // graph of one chunk: kernels only.  The key holds every pointer and
+  // launch parameter the captured kernels take, so a graph is never replayed
+  // against a matrix, workspace or vector that has been reallocated.
+  const PmDevMatrix &A = c->A;
+  const uintptr_t key[PmBatch::kGraphKey] = {
+      (uintptr_t)solver,    (uintptr_t)use32,     (uintptr_t)x,         (uintptr_t)m,
+      (uintptr_t)c->d_wk[0], (uintptr_t)c->d_wk[1], (uintptr_t)c->d_wk[2], (uintptr_t)c->d_wk[3],
+      (uintptr_t)A.val,     (uintptr_t)A.val32,   (uintptr_t)A.fr,      (uintptr_t)A.nr,
+      (uintptr_t)A.colptr,  (uintptr_t)A.valptr,  (uintptr_t)A.ncol,    (uintptr_t)A.chptr,
+      (uintptr_t)A.chunk,   (uintptr_t)A.neq,     (uintptr_t)B->G,
+      (uintptr_t)c->spmv_cfg[use32 ? 1 : 0]};
   if (B->use_graph && (B->graph == nullptr || memcmp(key, B->graph_key, sizeof(key)) != 0)) {
     if (B->graph) (void)hipGraphExecDestroy(B->graph);
     B->graph = nullptr;

@@ -168,6 +168,14 @@ extern "C" void pm_ctx_destroy(pm_ctx *c) {
 // matrix pattern on the device
 // ---------------------------------------------------------------------------
 int pm_upload_pattern(pm_ctx *c, const PmHostPattern &p) {
+  // a batch definition (box ranges, workgroup assignment, captured graph)
+  // refers to the matrix it was defined on: it goes with that matrix
+  if (c->batch) {
+    (void)hipStreamSynchronize(c->stream);
+    pm_batch_free(c->batch);
+    c->batch = nullptr;
+  }
+  FREE(c->d_eq_box);
   free_matrix(c->A);
   PmDevMatrix &A = c->A;
   A.nrn = p.nrn;

@@ -123,3 +123,74 @@ def test_batched_fp32ir_straggler_fallback():
     assert info["spmv64"] > info["outer"], info  # FP64 fallback products beyond the residuals
     assert info["true_rel"] < 1e-10
     assert np.abs(x32 - x64).max() <= 1e-8 * np.abs(x64).max()
+
+
+def test_batched_criterion2():
+    """Criterion 2 (Residuo.f:84-136) run per box in the batching form: the
+    load step converges in every box at the oracle's nonlinear iteration
+    (3), with each box's state within 1e-9 of the box stepped alone; the next
+    step reaches maxnlit in every box, as the oracle does (its -10), and the
+    batched step returns PM_ERR_NLIT."""
+    from oracle import oracle as O
+    from poromec_amd._lib import PmError
+    from poromec_amd.solver import MefSolver, SolverConfig
+    decks, big, off = _boxes()
+    cfg = SolverConfig(tol=1e-10, stop_crit=2, maxnlit=4)
+    for d in decks:
+        m = O.OracleModel(d, stop_crit=2, maxnlit=4)
+        assert [(i["rc"], i["nlit"]) for i in (m.step(check=False), m.step(check=False))] == \
+            [(0, 3), (-10, 4)]
+    sb = MefSolver(big)
+    sb.batch_define(off)
+    info = sb.step(cfg)
+    assert info["nlit"] == 3, info
+    singles = [MefSolver(d) for d in decks]
+    for s in singles:
+        assert s.step(cfg)["nlit"] == 3
+    parts = _split(sb.state()["u"], decks)
+    for k, s in enumerate(singles):
+        u = s.state()["u"]
+        nv = decks[k].nnodev
+        assert np.abs(parts[k][:, :3] - u[:, :3]).max() <= 1e-9 * np.abs(u[:, :3]).max()
+        assert np.abs(parts[k][:nv, 3] - u[:nv, 3]).max() <= 1e-9 * np.abs(u[:nv, 3]).max()
+    with pytest.raises(PmError) as e:
+        sb.step(cfg)
+    assert e.value.code == -7
+    for s in singles:
+        with pytest.raises(PmError) as e:
+            s.step(cfg)
+        assert e.value.code == -7
+        s.close()
+    sb.close()
+
+
+def test_resetup_drops_the_batch():
+    """A new mesh on a context that held a batch definition drops it (the box
+    ranges and the captured graph refer to the old matrix): afterwards the
+    context solves as one system and the batch queries report PM_ERR_STATE."""
+    from poromec_amd import _lib as L
+    from poromec_amd.solver import MefSolver, SolverConfig
+    import ctypes as C
+    decks, big, off = _boxes()
+    sb = MefSolver(big)
+    sb.batch_define(off)
+    sb.step(SolverConfig(tol=1e-10))
+    d = decks[1]
+    ix, x, idv = L.as_i32(d.ix), L.as_f64(d.x), L.as_i32(d.idv)
+    nnode, neq, nad = C.c_int(), C.c_int(), C.c_int64()
+    L.check(sb.lib.pm_mesh_setup(sb.ctx, d.numel, d.nnodev, L.ptr(ix), L.ptr(x), L.ptr(idv), 1,
+                                 C.byref(nnode), C.byref(neq), C.byref(nad)), "pm_mesh_setup")
+    it = np.zeros(sb.nbox, np.int32)
+    assert sb.lib.pm_batch_iters(sb.ctx, L.ptr(it)) == -8
+    sb.deck, sb.nnode, sb.neq = d, nnode.value, neq.value
+    z4, zn = np.zeros((sb.nnode, 4)), np.zeros(sb.nnode)
+    sb.set_state(u=z4, u0=z4, dp=zn, t=0.0)
+    sb.set_materials(d)
+    info = sb.step(SolverConfig(tol=1e-10))
+    one = MefSolver(d)
+    ref = one.step(SolverConfig(tol=1e-10))
+    assert abs(info["iters"][0] - ref["iters"][0]) <= 2 + 0.1 * ref["iters"][0]
+    u, u1 = sb.state()["u"], one.state()["u"]
+    assert np.abs(u - u1).max() <= 1e-9 * np.abs(u1).max()
+    one.close()
+    sb.close()

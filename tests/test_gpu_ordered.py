@@ -213,3 +213,61 @@ def test_tform_bitwise(which):
         assert np.array_equal(g, o), (name, np.abs(g - o).max())
         assert np.abs(o[: d.nnodev]).max() > 0
     s.close()
+
+
+def test_ordered_march_criterion2():
+    """cal_residuo_pm criterion 2 (Residuo.f:84-136, the reference default
+    with the resp0 = max(resp0, gresu) quirk of line 119) on the GPU in
+    reference order: the load step converges at the oracle's nonlinear
+    iteration with its Krylov counts and state bit for bit; the next step
+    reaches maxnlit like the oracle (PM_ERR_NLIT for its -10, the state at
+    that point equal bit for bit), and with maxnlit_accept both keep it."""
+    from poromec_amd._lib import PmError
+    from poromec_amd.solver import MefSolver, SolverConfig
+    d = P.column_deck(2, 2, 8, dt=0.01, nsteps=6)
+    st = O.build_structure(d)
+    for accept in (False, True):
+        cfg = SolverConfig(stop_crit=2, maxnlit=4, maxnlit_accept=accept)
+        m = O.OracleModel(d, st, stop_crit=2, maxnlit=4, maxnlit_accept=accept)
+        s = MefSolver(d)
+        s.set_ordered(True)
+        io = m.step(check=False)
+        ig = s.step(cfg)
+        assert io["rc"] == 0 and io["nlit"] == 3, io
+        assert (io["nlit"], list(io["iters"])) == (ig["nlit"], ig["iters"]), (io, ig)
+        g = s.state()
+        assert np.array_equal(g["u"], m.u) and np.array_equal(g["dp"], m.dp)
+        io = m.step(check=False)
+        assert io["nlit"] == 4
+        if accept:
+            assert io["rc"] == 0
+            ig = s.step(cfg)
+            assert (io["nlit"], list(io["iters"])) == (ig["nlit"], ig["iters"]), (io, ig)
+        else:
+            assert io["rc"] == -10
+            with pytest.raises(PmError) as e:
+                s.step(cfg)
+            assert e.value.code == -7  # PM_ERR_NLIT
+        g = s.state()
+        assert np.array_equal(g["u"], m.u), np.abs(g["u"] - m.u).max()
+        assert np.array_equal(g["dp"], m.dp)
+        s.close()
+
+
+def test_ordered_sqmr_gmres_fallback(saddle):
+    """solv_pm's fallback (Solv.f:92-94,765; Iters.f:6343-6349): SQMR stopped
+    at maxit hands the system to GMRES(ngram) from x = 0 under the same maxit.
+    In reference order the GPU reports GMRES as the finishing solver, takes
+    the oracle's GMRES iterations after its maxit SQMR ones, and returns the
+    oracle's solution bit for bit."""
+    from poromec_amd.solver import SolverConfig
+    st, m = saddle
+    b = np.random.default_rng(9).standard_normal(st["neq"])
+    it_o, x_o, used_o = O.solv(st, m.ad, m.al[: st["nad"]], b, solver=11, tol=1e-10, maxit=5)
+    assert used_o == 2 and it_o > 0
+    cs = _csrc(st, m)
+    x, info = cs.solve(b, SolverConfig(solver="sqmr", tol=1e-10, maxit=5))
+    cs.close()
+    assert info["used"] == 2
+    assert info["iters"] == 5 + it_o, (info["iters"], it_o)
+    assert np.array_equal(x, x_o), np.abs(x - x_o).max() / np.abs(x_o).max()

@@ -225,3 +225,26 @@ def test_time_march_histories_heterogeneous():
     d = P.heterogeneous_box_deck(4, seed=7, dt=0.1, nsteps=4, corr_len=2.0)
     eu, ep, its = _history_errors(d, 4)
     assert eu < 1e-9 and ep < 1e-9, (eu, ep, its)
+
+
+def test_sqmr_gmres_fallback_fast_mode(column):
+    """The SQMR -> GMRES fallback on the assembled device matrix in the
+    default (fast) mode: GMRES finishes the solve and reaches the tolerance
+    the oracle's fallback reaches."""
+    from poromec_amd.solver import SolverConfig
+    d, st = column
+    s = _solver(d)
+    s.pform(2, True, False, d.dt, d.dt)
+    m = O.OracleModel(d, st)
+    m.assemble(isw=2, lhs=True, rhs=False)
+    b = np.random.default_rng(6).standard_normal(st["neq"])
+    it_o, x_o, used_o = O.solv(st, m.ad, m.al[: st["nad"]], b, solver=11, tol=1e-10, maxit=5)
+    assert used_o == 2 and it_o > 0
+    x, info = s.solve(b, SolverConfig(solver="sqmr", tol=1e-10, maxit=5))
+    assert info["used"] == 2
+    assert abs((info["iters"] - 5) - it_o) <= 2 + 0.1 * it_o, (info["iters"], it_o)
+    # GMRES stops on the left-Jacobi-preconditioned residual (Iters.f:1411-1775)
+    minv = 1.0 / m.ad
+    r = minv * (b - s.spmv(x))
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(minv * b)
+    assert np.abs(x - x_o).max() <= 1e-6 * np.abs(x_o).max()
