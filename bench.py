@@ -492,12 +492,21 @@ def main():
         ms32 = solver.bench_spmv(True, args.spmv_reps)
         copy_bytes = 1 << 30
         msc = solver.bench_copy(copy_bytes, args.spmv_reps)
+        sbytes = 4 << 30
+        stream = {}
+        for kind, name, moved in ((0, "copy_32B_gridstride", 2), (1, "read_4x16B", 1),
+                                  (2, "read_4x16B_nontemporal", 1), (3, "copy_4x16B", 2),
+                                  (4, "write_4x16B", 1), (5, "read_8x16B", 1)):
+            ms = solver.bench_stream(kind, sbytes, args.spmv_reps)
+            stream[name] = {"ms": ms, "GBps": moved * sbytes / ms / 1e6}
+            log(f"stream {name}: {ms:.3f} ms, {moved * sbytes / ms / 1e6:.0f} GB/s")
         print(json.dumps({"spmv_fp64_ms": ms64, "spmv_fp32_ms": ms32,
                           "spmv_fp64_bytes": spmv_bytes(info, 8, shape=shapes["fp64"]["best"]),
                           "spmv_fp32_bytes": spmv_bytes(info, 4, shape=shapes["fp32"]["best"]),
                           "copy_ms": msc,
                           "copy_bytes_read": copy_bytes, "copy_bytes_written": copy_bytes,
                           "copy_GBps": 2 * copy_bytes / msc / 1e6, "tuned_shapes": shapes,
+                          "stream_4GiB": stream,
                           **info}), flush=True)
         return
 
@@ -517,24 +526,25 @@ def main():
         z4, zn = np.zeros((solver.nnode, 4)), np.zeros(solver.nnode)
         solver.set_state(u=z4, u0=z4, dp=zn, porosity=zn, t=0.0)
 
-    def march(cfg, max_steps, warmup, reserve_after):
-        """Untimed warmup steps, then timed steps from the zero state while the
-        next one (and reserve(k, est) seconds of later work after k steps of
-        est seconds each) still fit the budget: always at least one.  Each
-        step is timed alone (pm_step returns after the device has finished
-        it); the state is read back after each timed step, outside its time."""
+    def march(cfg, max_steps, warmup, more_warmup, more_timed):
+        """From the zero state: up to `warmup` untimed steps, then timed steps,
+        each next one only while more_warmup(w, est) / more_timed(w, k, est)
+        says the rest of the run still fits the budget (est: the seconds of
+        one step so far); always at least one timed step.  Warmup steps are
+        the march's first steps, so the timed ones are the steps after them.
+        Each step is timed alone (pm_step returns after the device has
+        finished it); the state is read back after each timed step, outside
+        its time."""
         zero_state()
-        done_w = 0
-        for k in range(warmup):
+        done_w, est = 0, None
+        while done_w < warmup:
             t1 = time.perf_counter()
             inf = solver.step(cfg)
             est = time.perf_counter() - t1
             done_w += 1
-            log(f"  {cfg.precision} warmup step {k}: {est:.1f} s, iters {inf['iters']}")
-            if left() - margin < 2 * est + reserve_after(1, est):
+            log(f"  {cfg.precision} warmup step {done_w - 1}: {est:.1f} s, iters {inf['iters']}")
+            if done_w < warmup and not more_warmup(done_w, est):
                 break
-        if done_w:
-            zero_state()
         solver.spmv_timing(args.live_every)  # HIP events around every k-th solver SpMV
         walls, infos, states = [], [], []
         while len(walls) < max_steps:
@@ -547,22 +557,26 @@ def main():
             log(f"  {cfg.precision} timed step {len(walls) - 1}: {walls[-1]:.1f} s, "
                 f"iters {inf['iters']}, true_rel {max(inf['true_rel']):.2e}")
             est = sum(walls) / len(walls)
-            if left() - margin < est + reserve_after(len(walls) + 1, est):
+            if not more_timed(done_w, len(walls), est):
                 break
         live = solver.spmv_timing(0)
         return done_w, walls, infos, states, live
 
-    # FP32+IR first: it is `value`.  Its steps leave room for as many FP64
-    # steps (estimated at 1.7x an FP32+IR step until one is measured) and
-    # for the CPU leg.
+    # FP32+IR first: it is `value`.  Its steps leave room for the FP64 form's
+    # same warmup and at least as many timed steps (an FP64 step estimated at
+    # 1.8 FP32+IR steps until one is measured) and for the CPU leg.
     cpu_reserve = 0.0 if args.no_cpu else args.cpu_reserve
+    r64 = 1.8
     w32, walls32, infos32, st32, live32 = march(
-        cfg32, args.steps, args.warmup, lambda k, est: 1.7 * est * k + cpu_reserve)
+        cfg32, args.steps, args.warmup,
+        lambda w, est: left() - margin >= 2 * est + r64 * est * (w + 2) + cpu_reserve,
+        lambda w, k, est: left() - margin >= est + r64 * est * (w + k + 1) + cpu_reserve)
     est32 = sum(walls32) / len(walls32)
     walls64, infos64, st64, live64, w64 = [], [], [], None, 0
-    if left() - margin >= 1.7 * est32 + cpu_reserve:
+    if left() - margin >= r64 * est32 * (w32 + 1) + cpu_reserve:
         w64, walls64, infos64, st64, live64 = march(
-            cfg64, len(walls32), w32, lambda k, est: cpu_reserve)
+            cfg64, len(walls32), w32, lambda w, est: True,
+            lambda w, k, est: left() - margin >= est + cpu_reserve)
     else:
         log(f"FP64 form skipped: {left():.0f} s left of the {budget:.0f} s budget")
 

@@ -285,6 +285,16 @@ int pm_spmv_timing(pm_ctx *ctx, int on, double *stats);
  * bytes, writes bytes): the reference point for HBM traffic counters. */
 int pm_bench_copy(pm_ctx *ctx, int64_t bytes, int reps, double *ms_per_launch);
 
+/* Average device time of a streaming kernel over `bytes` bytes (no
+ * counterpart in the reference): what HBM delivers on this device, the
+ * reference point of the SpMV's bandwidth figures.  kind 0: copy, 32 B per
+ * lane, grid-stride (pm_bench_copy's kernel); 1: read-only, 4 x 16 B in
+ * flight per lane; 2: the same with nontemporal loads; 3: copy, 4 x 16 B per
+ * lane; 4: write-only, 4 x 16 B per lane; 5: read-only, 8 x 16 B per lane.
+ * Kinds 1-5 launch one workgroup of 256 lanes per 16 KB (32 KB for kind 5).
+ * A copy moves `bytes` in and `bytes` out; a read or a fill `bytes` once. */
+int pm_bench_stream(pm_ctx *ctx, int kind, int64_t bytes, int reps, double *ms_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,7 @@ def lib():
     L.pm_matrix_runs.argtypes = [_vp, C.POINTER(C.c_int64)]
     L.pm_bench_spmv.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
     L.pm_bench_copy.argtypes = [_vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    L.pm_bench_stream.argtypes = [_vp, C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]
     _lib = L
     return L
 

@@ -40,6 +40,12 @@ class SolverConfig:
     ir_maxouter: int = 10
     maxnlit: int = 500
     nltol: float = 1.0e-4
+    # The reference's nl macro defaults to criterion 2 (Main.f:273), and
+    # read_deck keeps that default for decks.  Criterion 2 freezes the
+    # displacement scale at nli 2 (Residuo.f:117-119); after a load step the
+    # displacement residual is round-off from nli 1 on, so such a step never
+    # converges and the reference stops at maxnlit.  Marches built in code
+    # therefore default to criterion 1 (one norm over all equations).
     stop_crit: int = 1
     maxnlit_accept: bool = False  # keep the step at maxnlit instead of PM_ERR_NLIT
 
@@ -277,6 +283,13 @@ class MefSolver:
     def bench_copy(self, nbytes: int, reps: int = 20) -> float:
         ms = C.c_double()
         L.check(self.lib.pm_bench_copy(self.ctx, int(nbytes), reps, C.byref(ms)), "pm_bench_copy")
+        return ms.value
+
+    def bench_stream(self, kind: int, nbytes: int, reps: int = 20) -> float:
+        """ms per launch of streaming kernel `kind` over nbytes (pm_bench_stream)."""
+        ms = C.c_double()
+        L.check(self.lib.pm_bench_stream(self.ctx, int(kind), int(nbytes), reps, C.byref(ms)),
+                "pm_bench_stream")
         return ms.value
 
     def close(self):
