@@ -167,6 +167,27 @@ int pm_pform(pm_ctx *ctx, int isw, int lhs, int rhs, double dt, double t, int ta
 int pm_solve(pm_ctx *ctx, const pm_solver_cfg *cfg, const double *b, double *x,
              pm_solve_info *info);
 
+/* ---- pm_solve without allocating (no counterpart in the reference: it is
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops) ----
+ * The same solve as pm_solve, with its scratch (Krylov work vectors, the
+ * Jacobi vector, reduction partials, device copies of b and x) in a static
+ * device array of the library, (19 * PM_NOALLOC_MAX_NEQ + 32768) doubles
+ * (80 MB), that each GPU holds from the library's first use on it until
+ * the process ends.  It calls no hipMalloc or hipFree.  The matrix must be
+ * set up before (pm_mesh_setup + pm_pform, or pm_struct_upload +
+ * pm_values_upload).  Limits: neq <= PM_NOALLOC_MAX_NEQ (PM_ERR_ARG beyond),
+ * not GMRES and no SQMR -> GMRES fallback (SQMR at maxit returns
+ * PM_ERR_MAXIT), not the batching form (PM_ERR_UNSUPPORTED), FP32IR only
+ * once the FP32 values exist (PM_ERR_STATE otherwise; a pm_solve with
+ * FP32IR or pm_tune_spmv(use32 = 1) makes them).  b / x as in pm_solve; NULL
+ * needs the context's device vectors to exist (PM_ERR_STATE otherwise).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU race. */
+#define PM_NOALLOC_MAX_NEQ 524288
+int pm_solve_noalloc(pm_ctx *ctx, const pm_solver_cfg *cfg, const double *b, double *x,
+                     pm_solve_info *info);
+
 /* ---- drop-in solv_pm with the caller's CSRC matrix ----
  * Uploads the strict-lower CSRC pattern (ia, ja: Datastruct.f:124-150) and
  * values (ad, al), then solves A x = b with A = diag(ad) + L + L^T.

@@ -79,6 +79,7 @@ def lib():
     L.pm_set_nodal_loads.argtypes = [_vp, _vp, _vp]
     L.pm_pform.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
     L.pm_solve.argtypes = [_vp, C.POINTER(SolverCfg), _vp, _vp, C.POINTER(SolveInfo)]
+    L.pm_solve_noalloc.argtypes = [_vp, C.POINTER(SolverCfg), _vp, _vp, C.POINTER(SolveInfo)]
     L.pm_struct_upload.argtypes = [_vp, C.c_int, C.c_int64, _vp, _vp]
     L.pm_values_upload.argtypes = [_vp, _vp, _vp]
     L.pm_spmv.argtypes = [_vp, _vp, _vp]

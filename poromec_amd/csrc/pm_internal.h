@@ -205,6 +205,10 @@ struct pm_ctx {
   // reference summation order: every reduction a sequential left-to-right
   // sum, the SpMV in the order of matvec_csrc_sym_pm (slow; for parity)
   int ordered = 0;
+  // set for the duration of pm_solve_noalloc: the solve must not allocate,
+  // so SQMR reaching maxit returns PM_ERR_MAXIT instead of falling back to
+  // GMRES (whose basis would need (ngram + 1) more vectors)
+  bool no_alloc = false;
 
   // batching form
   PmBatch *batch = nullptr;

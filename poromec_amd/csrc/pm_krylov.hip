@@ -1306,6 +1306,7 @@ static int krylov(pm_ctx *c, const pm_solver_cfg *cfg, int n, const double *b, d
     case PM_SOLVER_SQMR: {
       int failed = 0;
       PM_TRY(run_sqmr(c, n, b, x, m, tol, cfg->maxit, use32, info, &failed));
+      if (failed && c->no_alloc) return PM_ERR_MAXIT;
       if (failed) {
         used = PM_SOLVER_GMRES;
         PM_TRY(run_gmres(c, n, b, x, m, cfg->nkrylov, tol, cfg->maxit, use32, info));
