@@ -424,6 +424,9 @@ def main():
                     help="seconds kept free at the end of the budget")
     ap.add_argument("--cpu-reserve", type=float, default=60.0,
                     help="seconds the CPU baseline leg is allowed for")
+    ap.add_argument("--only", choices=["fp32ir", "fp64"], default=None,
+                    help="time one form only (the 128^3 box: a step of each takes minutes); "
+                         "--only fp64 reports the FP64 form's time steps per second as value")
     ap.add_argument("--live-every", type=int, default=16,
                     help="time every k-th solver SpMV with HIP events")
     ap.add_argument("--n", type=int, default=87, help="box edge in cells")
@@ -566,14 +569,18 @@ def main():
     # same warmup and at least as many timed steps (an FP64 step estimated at
     # 1.8 FP32+IR steps until one is measured) and for the CPU leg.
     cpu_reserve = 0.0 if args.no_cpu else args.cpu_reserve
-    r64 = 1.8
+    r64 = 0.0 if args.only == "fp32ir" else 1.8
+    if args.only == "fp64":  # the FP64 form stands in the FP32+IR form's place
+        cfg32 = cfg64
     w32, walls32, infos32, st32, live32 = march(
         cfg32, args.steps, args.warmup,
         lambda w, est: left() - margin >= 2 * est + r64 * est * (w + 2) + cpu_reserve,
         lambda w, k, est: left() - margin >= est + r64 * est * (w + k + 1) + cpu_reserve)
     est32 = sum(walls32) / len(walls32)
     walls64, infos64, st64, live64, w64 = [], [], [], None, 0
-    if left() - margin >= r64 * est32 * (w32 + 1) + cpu_reserve:
+    if args.only is not None:
+        log(f"only the {args.only} form timed")
+    elif left() - margin >= r64 * est32 * (w32 + 1) + cpu_reserve:
         w64, walls64, infos64, st64, live64 = march(
             cfg64, len(walls32), w32, lambda w, est: True,
             lambda w, k, est: left() - margin >= est + cpu_reserve)
@@ -595,9 +602,11 @@ def main():
     # stream around every live_every-th solver SpMV), per value precision
     b32 = spmv_bytes(info, 4, fused_dot=True, shape=shapes["fp32"]["best"])
     b64 = spmv_bytes(info, 8, fused_dot=True, shape=shapes["fp64"]["best"])
+    if args.only == "fp64":  # the timed form's SpMV is the FP64 one
+        b32, live32 = b64, {"fp32_ms": live32["fp64_ms"], "fp32_n": live32["fp64_n"]}
     ms32 = live32["fp32_ms"] / max(live32["fp32_n"], 1)
     ms64 = live64["fp64_ms"] / max(live64["fp64_n"], 1) if live64 else None
-    alg32 = spmv_alg_bytes(info, 4)
+    alg32 = spmv_alg_bytes(info, 8 if args.only == "fp64" else 4)
     alg64 = spmv_alg_bytes(info, 8)
     log(f"SpMV in the timed steps (every {args.live_every}th launch timed): fp32 {ms32:.3f} ms x "
         f"{live32['fp32_n']}" + (f", fp64 {ms64:.3f} ms x {live64['fp64_n']}" if ms64 else ""))
@@ -626,7 +635,8 @@ def main():
         except Exception as exc:  # the oracle is test infrastructure: optional here
             log(f"cpu baseline skipped: {exc}")
 
-    traffic32 = measured_traffic("fp32", shapes["fp32"]["best"], info)
+    traffic32 = (measured_traffic("fp64", shapes["fp64"]["best"], info) if args.only == "fp64"
+                 else measured_traffic("fp32", shapes["fp32"]["best"], info))
     out = {
         "metric": METRIC,
         "value": 1e3 / ms_step32,
@@ -638,31 +648,40 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-spmv/f64-ir",
+        "dtype": "f64" if args.only == "fp64" else "f32-spmv/f64-ir",
         "data": "synthetic",
         "config": {
-            "workload": (f"Biot u-p hexa20/8 box {n}^3 cells ({solver.neq} equations, the "
-                         f"metric's 8.6M-dof size), heterogeneous log-k, 10% patch load, "
-                         f"dt 0.1 s, SQMR+Jacobi 1e-10, FP32+IR"),
+            "workload": (f"Biot u-p hexa20/8 box {n}^3 cells ({solver.neq} equations"
+                         + (", the metric's 8.6M-dof size" if n == 87 else "")
+                         + f"), heterogeneous log-k, 10% patch load, dt 0.1 s, SQMR+Jacobi "
+                         f"1e-10, {'FP64' if args.only == 'fp64' else 'FP32+IR'}"),
             "n": n, "numel": deck.numel, "nnode": solver.nnode, "neq": solver.neq,
             "nnz": info["nnz"], "dt": deck.dt, "solver": "sqmr", "precond": "diag",
             "tol": 1e-10, "setup_s": round(setup_s, 2),
         },
         "requested": {"steps": args.steps, "warmup": args.warmup, "budget_s": budget},
-        "fp32ir": {"time_steps_per_s": 1e3 / ms_step32, "ms_per_step": ms_step32,
+        "fp32ir": None if args.only == "fp64" else {"time_steps_per_s": 1e3 / ms_step32, "ms_per_step": ms_step32,
                    "inner_iters_per_step": it32,
                    "krylov_iters_per_s": it32 / (ms_step32 / 1e3), "max_true_rel": rel32,
                    "max_rel_err_vs_fp64": {"u": err_u, "p": err_p},
                    "spmv_ms": ms32, "spmv_launches_timed": live32["fp32_n"],
                    "spmv_GBps_algorithmic": alg32 / ms32 / 1e6,
                    "spmv_GBps_compulsory": b32 / ms32 / 1e6},
-        "fp64": fp64,
+        "fp64": fp64 if args.only != "fp64" else {
+            "time_steps_per_s": 1e3 / ms_step32, "ms_per_step": ms_step32, "steps": steps32,
+            "warmup": w32, "krylov_iters_per_step": it32,
+            "krylov_iters_per_s": it32 / (ms_step32 / 1e3), "max_true_rel": rel32,
+            "spmv_ms": ms32, "spmv_launches_timed": live32["fp32_n"],
+            "spmv_GBps_algorithmic": alg32 / ms32 / 1e6, "spmv_GBps_compulsory": b32 / ms32 / 1e6},
         "speedup_fp32ir_vs_fp64": (fp64["ms_per_step"] / ms_step32) if fp64 else None,
         "spmv": {"tuned_shapes": shapes, "fp32_alg_bytes": alg32, "fp64_alg_bytes": alg64,
                  "fp32_compulsory_bytes": b32, "fp64_compulsory_bytes": b64},
         "roofline": {"bound": "hbm",
-                     "kernel": spmv_kernel_name("fp32", shapes["fp32"]["best"])
-                               + " (FP32 node-block SpMV with the fused q.t dot)",
+                     "kernel": (spmv_kernel_name("fp64", shapes["fp64"]["best"])
+                                + " (FP64 node-block SpMV with the fused q.t dot)")
+                     if args.only == "fp64" else
+                     (spmv_kernel_name("fp32", shapes["fp32"]["best"])
+                      + " (FP32 node-block SpMV with the fused q.t dot)"),
                      "achieved": alg32 / ms32 / 1e6, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": alg32 / ms32 / 1e6 / HBM_PEAK_GBPS,
                      "traffic": traffic32,

@@ -11,6 +11,9 @@
 // product is a deterministic two-level reduction whose last workgroup also
 // computes the next scalars, so the host only polls a done flag.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -586,6 +589,22 @@ static int poll_done(pm_ctx *c, int *done, int *iter) {
   PM_CHECK_HIP(hipStreamSynchronize(c->stream));
   *done = c->h_scal->done;
   *iter = c->h_scal->iter;
+  // PM_PROGRESS_SECONDS=s (opt-in): the solve's iteration and recurrence
+  // residual on stderr every s seconds (long solves under a watchdog)
+  static const double progress_s = [] {
+    const char *e = std::getenv("PM_PROGRESS_SECONDS");
+    return e ? std::atof(e) : 0.0;
+  }();
+  if (progress_s > 0.0) {
+    static thread_local auto last = std::chrono::steady_clock::now();
+    const auto now = std::chrono::steady_clock::now();
+    if (std::chrono::duration<double>(now - last).count() >= progress_s) {
+      std::fprintf(stderr, "[pm_solve] %d iterations launched, residual %.3e\n", c->cur_iter,
+                   c->h_scal->resid);
+      std::fflush(stderr);
+      last = now;
+    }
+  }
   if (c->tev_used) pm_spmv_harvest(c, *done, *iter);
   return PM_OK;
 }

@@ -2,10 +2,16 @@
 
 The gradient of a sensor misfit over a march with respect to per-material
 log E and log permeability must match central finite differences of the
-same device march (relative error 1e-5 along random and coordinate
-directions, Krylov tolerance 1e-13 so that each step is one exact solve), and
-central finite differences of the oracle's own march (the FP64 reference
-gradient).
+same device march to 1e-6 relative along random and coordinate directions,
+and central finite differences of the oracle's own march (the FP64
+reference gradient) to the same bar.  Krylov tolerance 1e-13, so that each
+step is one exact solve.  The step h = 1e-5 comes from a sweep of the
+oracle's central differences on these decks (h = 1e-4 ... 1e-7): h = 1e-4
+carries ~6e-6 relative truncation error and h = 1e-5 about 1e-7; at h = 1e-5
+the differences are stable to <= 2e-7 on both decks, while at 1e-6 the
+gravity deck's differences (its misfit is a small difference of large
+gravity-driven states) move by up to 2e-6 from rounding.  The benchmark's
+config-5 test (test_gpu_configs.py) keeps the config's h = 1e-6.
 Checkpoint spacing must not change J, the gradients or the final state, and
 the torch.autograd wrapper must hand back the library's gradient."""
 import numpy as np
@@ -16,6 +22,8 @@ from poromec_amd import problem as P
 pytestmark = pytest.mark.gpu
 
 NSTEPS = 6
+FD_H = 1e-5    # central-difference step (see the module docstring)
+FD_REL = 1e-6  # the bar: relative error of the adjoint against the difference
 
 
 def _deck(gravity: bool):
@@ -70,7 +78,7 @@ def test_adjoint_matches_finite_differences(gravity):
     J, gE, gk = m._evaluate(lE, lk)
     assert J > 0 and np.all(np.isfinite(gE)) and np.all(np.isfinite(gk))
     rng = np.random.default_rng(3)
-    h = 1e-4
+    h = FD_H
     dirs = [rng.standard_normal(2 * len(lE)) for _ in range(2)]
     for ci in (int(np.argmax(np.abs(gE))), len(lE) + int(np.argmax(np.abs(gk)))):
         v = np.zeros(2 * len(lE))
@@ -83,7 +91,7 @@ def test_adjoint_matches_finite_differences(gravity):
         jm = m._evaluate(lE - h * vE, lk - h * vk)[0]
         fd = (jp - jm) / (2 * h)
         ad = float(g @ v)
-        assert abs(fd - ad) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(g).sum()), (fd, ad)
+        assert abs(fd) > 0 and abs(fd - ad) <= FD_REL * abs(fd), (fd, ad)
     m.close()
 
 
@@ -166,12 +174,12 @@ def test_adjoint_matches_oracle_finite_differences():
 
     assert abs(j_oracle(lE, lk) - J) <= 1e-8 * J
     rng = np.random.default_rng(21)
-    h = 1e-4
+    h = FD_H
     g = np.concatenate([gE, gk])
     for _ in range(2):
         v = rng.standard_normal(2 * len(lE))
         vE, vk = v[: len(lE)], v[len(lE):]
         fd = (j_oracle(lE + h * vE, lk + h * vk) - j_oracle(lE - h * vE, lk - h * vk)) / (2 * h)
         ad = float(g @ v)
-        assert abs(fd - ad) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(g).sum()), (fd, ad)
+        assert abs(fd) > 0 and abs(fd - ad) <= FD_REL * abs(fd), (fd, ad)
     m.close()

@@ -1,0 +1,168 @@
+"""The benchmark configurations on the GPU, at full size where the suite's
+time allows and otherwise at the largest size that fits it.
+
+* Config 1 (full size): the 4x4x40 consolidation column, E = 10 MPa,
+  nu = 0.3, k/mu = 0.1 m^2/(MPa s), M = 1000 MPa, 1 kPa step load, 20 steps of
+  dt = 0.01 s, SQMR + Jacobi to 1e-10: every step's displacement and
+  pressure within 1e-9 (relative to the field's largest entry) of the
+  oracle's march, the same nonlinear iterations, Krylov iterations within
+  the oracle's rounding spread, and the undrained response of Terzaghi's
+  problem (p0 = M q / (K_v + M) below the drained boundary layer).
+* Config 2 (full size): the 64^3 heterogeneous box (log-k field sigma 1,
+  correlation length 8 cells, E ~ U[5, 50] MPa, nu ~ U[0.25, 0.35], 1 kPa top
+  load, dt = 0.1 s): two FP64 steps meeting the 1e-10 true residual with the
+  restrained dofs still zero, and the FP32+IR step agreeing with the FP64
+  one to 1e-9.
+* Config 4 (16 boxes, n ~ U[12, 16] instead of 256 boxes, n ~ U[12, 32]): the
+  block-diagonal batched step against every box stepped alone.
+* Config 5 (an 8^3 box and 10 steps instead of 48^3 and 50): 64 pressure
+  sensors, targets from a second seeded field, checkpoints every 5 steps;
+  the adjoint gradient against central differences (h = 1e-6) along two
+  seeded directions to 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from poromec_amd import problem as P
+
+pytestmark = pytest.mark.gpu
+
+RTOL_HIST = 1e-9
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_config1_column_march():
+    from poromec_amd.solver import MefSolver, SolverConfig
+    d = P.column_deck(4, 4, 40, dt=0.01, nsteps=20)
+    st = O.build_structure(d)
+    m = O.OracleModel(d, st)
+    s = MefSolver(d)
+    assert (s.neq, s.nad) == (st["neq"], st["nad"])
+    nv = d.nnodev
+    E, nu, M, q = 10.0, 0.3, 1000.0, 1.0e-3
+    Kv = E * (1 - nu) / ((1 + nu) * (1 - 2 * nu))
+    p0 = M * q / (Kv + M)
+    z = d.x[:, 2]
+    deep = z <= z.max() - 5.0  # 5 m below the drained top
+    for k in range(20):
+        io = m.step()
+        ig = s.step(SolverConfig(tol=1e-10))
+        assert io["nlit"] == ig["nlit"], (k, io, ig)
+        for a, b in zip(io["iters"], ig["iters"]):
+            assert abs(a - b) <= 2 + 0.1 * max(a, b), (k, io["iters"], ig["iters"])
+        u = s.state()["u"]
+        assert _rel(u[:, :3], m.u[:, :3]) <= RTOL_HIST, (k, _rel(u[:, :3], m.u[:, :3]))
+        assert _rel(u[:nv, 3], m.u[:nv, 3]) <= RTOL_HIST, (k, _rel(u[:nv, 3], m.u[:nv, 3]))
+        # early consolidation: below the boundary layer the pore pressure is
+        # still the undrained p0 of Terzaghi's problem
+        assert np.abs(u[:nv, 3][deep] / p0 - 1.0).max() < 2e-3, k
+    s.close()
+
+
+def test_config2_64cube_fp64_and_fp32ir():
+    from poromec_amd.solver import MefSolver, SolverConfig
+    d = P.heterogeneous_box_deck(64, seed=0, dt=0.1, nsteps=100)
+    s = MefSolver(d)
+    s.set_spmv_shape(False, 14)
+    s.set_spmv_shape(True, 14)
+    _, idm, _ = s.numbering()
+    fixed = np.zeros((s.nnode, 4), bool)
+    fixed[:, :3] = idm[:, :3] == 0
+    fixed[: d.nnodev, 3] = idm[: d.nnodev, 3] == 0
+    cfg64 = SolverConfig(tol=1e-10)
+    states = []
+    for k in range(2):
+        info = s.step(cfg64)
+        assert info["nlit"] == 2 and max(info["true_rel"]) < 1e-10, (k, info)
+        u = s.state()["u"]
+        assert np.all(u[fixed] == 0.0)
+        assert np.abs(u[:, 2]).max() > 0 and np.abs(u[: d.nnodev, 3]).max() > 0
+        states.append(u.copy())
+    z4, zn = np.zeros((s.nnode, 4)), np.zeros(s.nnode)
+    s.set_state(u=z4, u0=z4, dp=zn, t=0.0)
+    s.set_materials(d)
+    info = s.step(SolverConfig(tol=1e-10, precision="fp32ir"))
+    assert max(info["true_rel"]) < 1e-10, info
+    u32, u64 = s.state()["u"], states[0]
+    nv = d.nnodev
+    assert _rel(u32[:, :3], u64[:, :3]) <= RTOL_HIST
+    assert _rel(u32[:nv, 3], u64[:nv, 3]) <= RTOL_HIST
+    s.close()
+
+
+def _split(u_all, decks):
+    nv_tot = sum(d.nnodev for d in decks)
+    out, vo, eo = [], 0, nv_tot
+    for d in decks:
+        c = P.box_counts(d.nx, d.ny, d.nz)
+        ne = c["nnode"] - c["nnodev"]
+        out.append(np.concatenate([u_all[vo: vo + d.nnodev], u_all[eo: eo + ne]]))
+        vo += d.nnodev
+        eo += ne
+    return out
+
+
+def test_config4_batch_16_boxes():
+    from poromec_amd.solver import MefSolver, SolverConfig
+    decks = P.batch_decks(nbox=16, seed=4, nmin=12, nmax=16)
+    big, off = P.concat_decks(decks)
+    sb = MefSolver(big)
+    sb.batch_define(off)
+    cfg = SolverConfig(tol=1e-10)
+    sb.step(cfg)
+    its = sb.batch_iters()
+    parts = _split(sb.state()["u"], decks)
+    for k, d in enumerate(decks):
+        one = MefSolver(d)
+        info = one.step(cfg)
+        a, b = info["iters"][0], int(its[k])
+        assert abs(a - b) <= 2 + 0.1 * max(a, b), (k, a, b)
+        u, nv = one.state()["u"], d.nnodev
+        assert _rel(parts[k][:, :3], u[:, :3]) <= RTOL_HIST, k
+        assert _rel(parts[k][:nv, 3], u[:nv, 3]) <= RTOL_HIST, k
+        one.close()
+    # the FP32+IR batched step from the same state meets the FP64 tolerance
+    z4, zn = np.zeros((sb.nnode, 4)), np.zeros(sb.nnode)
+    sb.set_state(u=z4, u0=z4, dp=zn, t=0.0)
+    sb.set_materials(big)
+    info = sb.step(SolverConfig(tol=1e-10, precision="fp32ir"))
+    assert max(info["true_rel"]) < 1e-10, info
+    sb.close()
+
+
+def test_config5_adjoint_8cube():
+    from poromec_amd.adjoint import MarchMisfit
+    from poromec_amd.solver import MefSolver, SolverConfig
+    n, nsteps, nsens, seed = 8, 10, 64, 5
+    cfg = SolverConfig(tol=1e-13, maxit=200000)
+    deck = P.heterogeneous_box_deck(n, seed=seed, dt=0.1, nsteps=nsteps, corr_len=n / 2)
+    tdeck = P.heterogeneous_box_deck(n, seed=seed + 100, dt=0.1, nsteps=nsteps, corr_len=n / 2)
+    rng = np.random.default_rng(seed + 1)
+    probe = MefSolver(deck)
+    _, idm, _ = probe.numbering()
+    free_p = np.nonzero(idm[: deck.nnodev, 3] > 0)[0]
+    sens = [(int(v), 3) for v in rng.choice(free_p, size=nsens, replace=False)]
+    probe.set_materials(tdeck)
+    obs = np.zeros((nsteps, nsens))
+    for k in range(nsteps):
+        probe.step(cfg)
+        u = probe.state()["u"]
+        obs[k] = [u[a, b] for a, b in sens]
+    probe.close()
+    m = MarchMisfit(deck, sens, obs, nsteps=nsteps, cfg=cfg, ckpt_every=5)
+    lE, lk = (t.numpy() for t in m.params())
+    J, gE, gk = m._evaluate(lE, lk)
+    assert J > 0
+    g = np.concatenate([gE, gk])
+    h = 1e-6
+    for _ in range(2):
+        v = rng.standard_normal(2 * len(lE))
+        vE, vk = v[: len(lE)], v[len(lE):]
+        fd = (m._evaluate(lE + h * vE, lk + h * vk)[0] - m._evaluate(lE - h * vE, lk - h * vk)[0]) / (2 * h)
+        ad = float(g @ v)
+        assert abs(fd) > 0 and abs(fd - ad) <= 1e-6 * abs(fd), (fd, ad)
+    m.close()

@@ -169,3 +169,19 @@ def test_initial_porosity_vertex_average():
         want = 0.3 if v in shared else (0.2 if v in d.ix[0, :8] else 0.4)
         assert abs(m.porosity[v - 1] - want) < 1e-15, v
     assert not m.porosity[d.nnodev:].any()
+
+
+def test_config1_column_undrained_response():
+    """The 4x4x40 consolidation column of the benchmark (dt = 0.01 s): in the
+    first steps the pore pressure 5 m or more below the drained top is the
+    undrained p0 = M q / (K_v + M) of Terzaghi's problem."""
+    E, nu, M, q = 10.0, 0.3, 1000.0, 1.0e-3
+    d = P.column_deck(4, 4, 40, E=E, nu=nu, dt=0.01, nsteps=20)
+    m = O.OracleModel(d)
+    Kv = E * (1 - nu) / ((1 + nu) * (1 - 2 * nu))
+    p0 = M * q / (Kv + M)
+    z = d.x[:, 2]
+    deep = z <= z.max() - 5.0
+    for _ in range(4):
+        m.step()
+        assert np.abs(m.u[: d.nnodev, 3][deep] / p0 - 1.0).max() < 2e-3
