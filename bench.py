@@ -211,13 +211,17 @@ def batch_form(args):
     t0 = time.perf_counter()
     s = MefSolver(big)
     s.batch_define(off)
+    if args.batch_eager:  # launch the iteration chunks one kernel at a time
+        s.batch_graphs(False)
     setup_s = time.perf_counter() - t0
     s.pform(2, True, False, big.dt, big.dt)
     shapes = {name: s.tune_spmv(use32, reps=5)[0] for use32, name in ((False, "fp64"), (True, "fp32"))}
     log(f"batch: {args.nbox} boxes, neq {s.neq}, setup {setup_s:.1f} s, shapes {shapes}")
     out = {}
-    for name, cfg in (("fp64", SolverConfig(tol=1e-10)),
-                      ("fp32ir", SolverConfig(tol=1e-10, precision="fp32ir"))):
+    forms = [f for f in (("fp64", SolverConfig(tol=1e-10)),
+                         ("fp32ir", SolverConfig(tol=1e-10, precision="fp32ir")))
+             if f[0] in args.batch_forms.split(",")]
+    for name, cfg in forms:
         z4, zn = np.zeros((s.nnode, 4)), np.zeros(s.nnode)
         s.set_state(u=z4, u0=z4, dp=zn, porosity=zn, t=0.0)
         for _ in range(args.warmup):
@@ -238,7 +242,9 @@ def batch_form(args):
     if args.seq_sample > 0:
         k = min(args.seq_sample, args.nbox)
         wall = 0.0
-        for d in decks[:k]:
+        for i, d in enumerate(decks[:k]):
+            if i % 32 == 0:
+                log(f"  one box at a time: box {i} of {k}, {wall:.1f} s of steps so far")
             one = MefSolver(d)
             cfg = SolverConfig(tol=1e-10)
             for _ in range(args.warmup):
@@ -248,9 +254,11 @@ def batch_form(args):
                 one.step(cfg)
             wall += time.perf_counter() - t0
             one.close()
-        seq = {"boxes_timed": k, "box_steps_per_s": k * args.steps / wall}
+        seq = {"boxes_timed": k, "box_steps_per_s": k * args.steps / wall,
+               "s_per_step_all_boxes": wall / max(args.steps, 1)}
         log(f"  one box at a time (fp64, first {k} boxes): {seq}")
-    print(json.dumps({"form": "batch", "nbox": args.nbox, "n_range": [args.nmin, args.nmax],
+    print(json.dumps({"form": "batch", "graphs": not args.batch_eager,
+                      "nbox": args.nbox, "n_range": [args.nmin, args.nmax],
                       "neq": s.neq, "steps": args.steps, "warmup": args.warmup,
                       "setup_s": setup_s, "spmv_shapes": shapes, **out,
                       "one_box_at_a_time_fp64": seq}), flush=True)
@@ -441,6 +449,11 @@ def main():
     ap.add_argument("--nmax", type=int, default=32)
     ap.add_argument("--seq-sample", type=int, default=8,
                     help="batch form: boxes also solved one at a time, for comparison")
+    ap.add_argument("--batch-forms", default="fp64,fp32ir",
+                    help="batch form: the forms stepped as one block-diagonal system")
+    ap.add_argument("--batch-eager", action="store_true",
+                    help="batch form: launch the iteration chunks kernel by kernel instead of "
+                         "replaying their captured graph")
     ap.add_argument("--nsteps", type=int, default=50, help="adjoint form: march length")
     ap.add_argument("--nsens", type=int, default=64)
     ap.add_argument("--ckpt", type=int, default=10)
