@@ -112,6 +112,23 @@ void pm_ctx_destroy(pm_ctx *ctx);
 int pm_mesh_setup(pm_ctx *ctx, int numel, int nnodev, const int *ix, const double *x,
                   const int *idv, int reorder, int *nnode, int *neq, int64_t *nad);
 
+/* The mesh macro for either element family, by the mesh's maxno (nen):
+ * nen = 20 is pm_mesh_setup (ix (21, numel), hexa8 input promoted to the
+ * hexa20/8 element); nen = 8 keeps a hexa8 mesh as read (ix (9, numel): 8
+ * vertices + material), as the reader sizes a maxno-8 mesh (nst = nenv *
+ * ndf = 32, Rdata.f:650-660), for the equal-order Q1/Q1 Biot element --
+ * element type 7, the ELMT07_PM the reference reserves without implementing
+ * it (Elmlib03.f:22-23).  Every node is then a vertex with u, v, w, p; the
+ * element is elmt17_pm's weak form with trilinear displacements and the
+ * 2 x 2 x 2 Gauss rule.  On such a mesh pm_set_materials takes element-7
+ * materials (same e(1..8)), pm_set_vprop accepts only ie = 7 with vprop off,
+ * and the reference summation order (pm_set_ordered) is refused by the
+ * element loop (PM_ERR_UNSUPPORTED); pm_get_numbering returns its ix in the
+ * (21, numel) layout with columns 9..20 zero. */
+int pm_mesh_setup_nen(pm_ctx *ctx, int nen, int numel, int nnodev, const int *ix,
+                      const double *x, const int *idv, int reorder, int *nnode, int *neq,
+                      int64_t *nad);
+
 /* The same promotion, restraints, ordering and numbering on the host only
  * (no device needed).  ix20 (21 x numel), id (4 x nnode_max) and perm
  * (nnode_max) receive the results; nnode_max >= nnodev + 12 * numel always
@@ -119,6 +136,12 @@ int pm_mesh_setup(pm_ctx *ctx, int numel, int nnodev, const int *ix, const doubl
 int pm_mesh_numbering(int numel, int nnodev, const int *ix, const int *idv, int reorder,
                       int nnode_max, int *ix20, int *id, int *perm, int *nnode, int *neq,
                       int64_t *nad);
+
+/* pm_mesh_numbering by the mesh's nen, as pm_mesh_setup_nen reads it (nen =
+ * 8: ix (9, numel), no promotion, nnode = nnodev; ix20 still (21, numel)). */
+int pm_mesh_numbering_nen(int nen, int numel, int nnodev, const int *ix, const int *idv,
+                          int reorder, int nnode_max, int *ix20, int *id, int *perm, int *nnode,
+                          int *neq, int64_t *nad);
 
 /* The promoted connectivity (21 x numel), equation numbers (4 x nnode) and
  * the RCM permutation (nnode, perm[i] = node at position i+1), any may be NULL. */

@@ -43,6 +43,7 @@ class _Model(C.Structure):
         ("u", C.c_void_p), ("u0", C.c_void_p), ("dp", C.c_void_p), ("porosity", C.c_void_p),
         ("b0", C.c_void_p), ("b", C.c_void_p), ("x0", C.c_void_p), ("m", C.c_void_p),
         ("ie", C.c_void_p), ("vprop", C.c_int * 4), ("vpropel", C.c_void_p),
+        ("nen", C.c_int),
     ]
 
 
@@ -69,6 +70,9 @@ def _load_lib():
     L = C.CDLL(str(path))
     L.orc_elmt17.restype = C.c_int
     L.orc_elmt17.argtypes = [_f64p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_int,
+                             C.POINTER(OracleEnv)]
+    L.orc_elmt07.restype = C.c_int
+    L.orc_elmt07.argtypes = [_f64p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_int,
                              C.POINTER(OracleEnv)]
     L.orc_promote_hexa20.restype = C.c_int
     L.orc_promote_hexa20.argtypes = [_i32p, C.c_int, C.c_int]
@@ -172,6 +176,25 @@ def elmt17(e, iqm, iqh, xl, ul, dpl, isw, env, p=None):
     return rc, p, s.reshape(NST, NST).T  # s stored column-major
 
 
+NST7 = 32
+
+
+def elmt07(e, iqm, iqh, xl, ul, dpl, isw, env, p=None):
+    """One call of the Q1/Q1 element (element type 7, nst 32); returns
+    (rc, p[120 for isw 3, else 32], s[32,32] as s[i, j])."""
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    iqm = np.ascontiguousarray(iqm, dtype=np.int32)
+    iqh = np.ascontiguousarray(iqh, dtype=np.int32)
+    xl = np.ascontiguousarray(xl, dtype=np.float64).reshape(-1)
+    ul = np.ascontiguousarray(ul, dtype=np.float64)
+    dpl = np.ascontiguousarray(dpl, dtype=np.float64)
+    n = 120 if isw == 3 else NST7
+    p = np.zeros(n) if p is None else np.ascontiguousarray(p, dtype=np.float64).copy()
+    s = np.zeros(NST7 * NST7)
+    rc = lib.orc_elmt07(e, iqm, iqh, xl, ul, dpl, p, s, isw, C.byref(env))
+    return rc, p, s.reshape(NST7, NST7).T  # s stored column-major
+
+
 def elmt18(e, iqm, iqh, xl, ul, dpl, vp, isw, env, p=None):
     """One elmt18_pm call; returns (rc, p, s[68,68] as s[i, j], vp after)."""
     e = np.ascontiguousarray(e, dtype=np.float64)
@@ -189,13 +212,21 @@ def elmt18(e, iqm, iqh, xl, ul, dpl, vp, isw, env, p=None):
 
 def build_structure(deck, reorder=True):
     """Runs promotion, mk_bound_quad, graph, RCM, numbering and the CSRC
-    pattern exactly as the mesh macro does.  Returns a dict of arrays."""
+    pattern exactly as the mesh macro does (a deck with nen = 8 -- maxno 8,
+    the Q1/Q1 element -- skips promotion and mid-edge restraints: every node
+    is a vertex).  Returns a dict of arrays."""
     numel, nnodev = deck.numel, deck.nnodev
+    nen = getattr(deck, "nen", NEN)
     ix = np.ascontiguousarray(deck.ix, dtype=np.int32).copy()
-    nnode = lib.orc_promote_hexa20(ix, numel, nnodev)
+    if nen == 8:
+        ix[:, 8:NEN] = 0
+        nnode = nnodev
+    else:
+        nnode = lib.orc_promote_hexa20(ix, numel, nnodev)
     idf = np.zeros((nnode, NDF), dtype=np.int32)
     idf[:nnodev] = deck.idv
-    lib.orc_bound_quad(idf, ix, numel)
+    if nen != 8:
+        lib.orc_bound_quad(idf, ix, numel)
     fnno = np.zeros(nnode, dtype=np.int32)
     fnno[:nnodev] = 1
     ipos = lib.orc_graph_count(ix, numel, nnode)
@@ -217,7 +248,7 @@ def build_structure(deck, reorder=True):
     lib.orc_csrc_fill(idf, perm, ip, ips, nnode, neq, ia, ja)
     return dict(ix=ix, nnode=nnode, nnodev=nnodev, numel=numel, idflags=idflags, id=idf,
                 fnno=fnno, ip=ip, ips=ips, perm=perm, neq=neq, nad=int(nad), ia=ia,
-                ja=ja[:nad])
+                ja=ja[:nad], nen=nen)
 
 
 class OracleModel:
@@ -254,6 +285,7 @@ class OracleModel:
         md = self._md
         md.numel, md.nnode, md.nnodev, md.neq, md.nad = (st["numel"], nnode, deck.nnodev, neq,
                                                          nad)
+        md.nen = st.get("nen", NEN)
         for name, arr in (("ix", st["ix"]), ("x", self.x), ("id", st["id"]), ("fnno", st["fnno"]),
                           ("e", self.e), ("eload", self.eload), ("eloadp", self.eloadp),
                           ("nload", self.nload), ("f", self.f), ("ia", st["ia"]),

@@ -35,29 +35,33 @@
  * stored and are dropped) */
 void orc_assemble_csr(const double *s, const double *p, const int *ld, const int64_t *ia,
                          const int *ja, double *al, double *ad, double *b, int neq, int lhs,
-                         int rhs) {
-  for (int i = 0; i < NST; i++) {
+                         int rhs, int nst) {
+  for (int i = 0; i < nst; i++) {
     int k = ld[i];
     if (k <= 0 || k > neq) continue;
     if (rhs) b[k - 1] = b[k - 1] - p[i];
     if (lhs) {
-      ad[k - 1] = ad[k - 1] + s[(size_t)i * NST + i];
-      for (int j = 0; j < NST; j++) {
+      ad[k - 1] = ad[k - 1] + s[(size_t)i * nst + i];
+      for (int j = 0; j < nst; j++) {
         int l = ld[j];
         if (l <= 0) continue;
         for (int64_t kk = ia[k - 1]; kk <= ia[k] - 1; kk++)
-          if (l == ja[kk - 1]) al[kk - 1] = al[kk - 1] + s[(size_t)j * NST + i];
+          if (l == ja[kk - 1]) al[kk - 1] = al[kk - 1] + s[(size_t)j * nst + i];
       }
     }
   }
 }
+
+static int model_nen(const orc_model *md) { return md->nen == 8 ? 8 : ORC_NEN; }
+int orc_nst(const orc_model *md) { return 3 * model_nen(md) + ORC_NENV; }
 
 /* gathers the element arrays exactly as pform_pm does */
 void orc_gather(const orc_model *md, int nel, int *ld, double *ul, double *dpl, double *xl,
                    double *pl) {
   const int *ix = md->ix + (size_t)nel * NEN1;
   int kk = 0;
-  for (int i = 0; i < ORC_NEN; i++) {
+  const int nen = model_nen(md);
+  for (int i = 0; i < nen; i++) {
     int no = ix[i];
     for (int j = 0; j < NDF - 1; j++) {
       ld[kk] = md->id[NDF * (no - 1) + j];
@@ -77,13 +81,17 @@ void orc_gather(const orc_model *md, int nel, int *ld, double *ul, double *dpl, 
   }
 }
 
-static int elem_type(const orc_model *md, int ma) { return md->ie ? md->ie[ma - 1] : 17; }
+static int elem_type(const orc_model *md, int ma) {
+  if (md->nen == 8) return 7;
+  return md->ie ? md->ie[ma - 1] : 17;
+}
 
 /* elmlib_pm: element nel of material ma */
 static int elmlib(orc_model *md, int nel, int ma, const int *iqm, const int *iqh,
                   const double *xl, const double *ul, const double *dpl, double *pl, double *sl,
                   int isw, const orc_env *env) {
   const double *el = md->e + (size_t)(ma - 1) * ORC_PROP;
+  if (elem_type(md, ma) == 7) return orc_elmt07(el, iqm, iqh, xl, ul, dpl, pl, sl, isw, env);
   if (elem_type(md, ma) == 18)
     return orc_elmt18(el, iqm, iqh, xl, ul, dpl, pl, sl, md->vpropel + (size_t)nel * 448, isw,
                       env);
@@ -105,7 +113,8 @@ int orc_pform(orc_model *md, int isw, int lhs, int rhs, double *bvec, const orc_
     rc = elmlib(md, nel, ma, md->eload + 7 * (size_t)nel, md->eloadp + 7 * (size_t)nel, xl, ul,
                 dpl, pl, sl, isw, env);
     if (rc) break;
-    orc_assemble_csr(sl, pl, ld, md->ia, md->ja, md->al, md->ad, bvec, md->neq, lhs, rhs);
+    orc_assemble_csr(sl, pl, ld, md->ia, md->ja, md->al, md->ad, bvec, md->neq, lhs, rhs,
+                     orc_nst(md));
   }
   free(sl);
   return rc;

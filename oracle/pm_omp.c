@@ -92,9 +92,14 @@ double omp_pform_sample(orc_model *md, int isw, int lhs, int rhs, double *bvec,
       orc_gather(md, nel, ld, ul, dpl, xl, pl);
       const int ma = md->ix[(size_t)nel * NEN1 + ORC_NEN];
       const double *el = md->e + (size_t)(ma - 1) * ORC_PROP;
-      orc_elmt17(el, md->eload + 7 * (size_t)nel, md->eloadp + 7 * (size_t)nel, xl, ul, dpl,
-                 pl, sl, isw, env);
-      orc_assemble_csr(sl, pl, ld, md->ia, md->ja, md->al, md->ad, bvec, md->neq, lhs, rhs);
+      if (md->nen == 8)
+        orc_elmt07(el, md->eload + 7 * (size_t)nel, md->eloadp + 7 * (size_t)nel, xl, ul, dpl,
+                   pl, sl, isw, env);
+      else
+        orc_elmt17(el, md->eload + 7 * (size_t)nel, md->eloadp + 7 * (size_t)nel, xl, ul, dpl,
+                   pl, sl, isw, env);
+      orc_assemble_csr(sl, pl, ld, md->ia, md->ja, md->al, md->ad, bvec, md->neq, lhs, rhs,
+                       orc_nst(md));
     }
     free(sl);
   }

@@ -65,6 +65,10 @@ typedef struct {
   const int *ie;      /* numat */
   int vprop[4];
   double *vpropel;    /* 7 x 64 x numel */
+  /* nodes per element: 20 (0 reads as 20) for the hexa20/8 elements of a
+   * maxno-20 mesh, 8 for the Q1/Q1 element of a maxno-8 mesh (ix columns
+   * 9..20 zero, every node a vertex with 4 dofs) */
+  int nen;
 } orc_model;
 
 typedef struct {
@@ -92,6 +96,11 @@ int orc_elmt17(const double *e, const int *iqm, const int *iqh, const double *xl
  * properties of the element's integration points (isw 10 updates them) */
 int orc_elmt18(const double *e, const int *iqm, const int *iqh, const double *xl,
                const double *ul, const double *dpl, double *p, double *s, double *vp, int isw,
+               const orc_env *env);
+/* elmt07_pm, the equal-order Q1/Q1 hexa8 element (nst 32; pm_elmt07.c),
+ * isw 2, 3, 4, 7, with the argument conventions of orc_elmt17 */
+int orc_elmt07(const double *e, const int *iqm, const int *iqh, const double *xl,
+               const double *ul, const double *dpl, double *p, double *s, int isw,
                const orc_env *env);
 double orc_konzey_carman(double perm0, double poro, double poro0);
 void orc_hashin_shtrikman(double poro, double ks, double mus, double *k, double *mu,
@@ -159,7 +168,9 @@ void orc_gather(const orc_model *md, int nel, int *ld, double *ul, double *dpl, 
                 double *pl);
 void orc_assemble_csr(const double *s, const double *p, const int *ld, const int64_t *ia,
                       const int *ja, double *al, double *ad, double *b, int neq, int lhs,
-                      int rhs);
+                      int rhs, int nst);
+/* element dofs of the model's element: 68 (hexa20/8) or 32 (Q1/Q1) */
+int orc_nst(const orc_model *md);
 
 /* ---- the reference's OpenMP CPU path (pm_omp.c), the timed CPU baseline ---- */
 int omp_colour(const orc_model *md, int *elcolor, int maxcolors);

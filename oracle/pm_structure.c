@@ -77,14 +77,23 @@ void orc_bound_quad(int *id, const int *ix, int numel) {
 }
 
 /* size of the raw neighbour list (aponta_v1 ipos) */
+/* nodes of element e inside 1..nnode (20 for hexa20; 8 for a maxno-8 mesh,
+ * whose columns 9..20 are zero) */
+static int elem_nodes(const int *ix, int e, int nnode) {
+  int n = 0;
+  for (int j = 1; j <= ORC_NEN; j++) {
+    int no = IX(j, e);
+    if (no != 0 && no <= nnode) n++;
+  }
+  return n;
+}
+
 int64_t orc_graph_count(const int *ix, int numel, int nnode) {
   int64_t ipos = 0;
-  for (int e = 0; e < numel; e++)
-    for (int j = 1; j <= ORC_NEN; j++) {
-      int no = IX(j, e);
-      if (no == 0 || no > nnode) continue;
-      ipos += ORC_NEN - 1;
-    }
+  for (int e = 0; e < numel; e++) {
+    int n = elem_nodes(ix, e, nnode);
+    ipos += (int64_t)n * (n - 1);
+  }
   return ipos;
 }
 
@@ -100,12 +109,14 @@ void orc_graph(const int *ix, int numel, int nnode, int64_t *ip, int *ips, int *
   int *cnt = calloc((size_t)nnode + 1, sizeof(int));
   memset(ip, 0, ((size_t)nnode + 1) * sizeof(int64_t));
   ip[0] = 1;
-  for (int e = 0; e < numel; e++)
+  for (int e = 0; e < numel; e++) {
+    int n = elem_nodes(ix, e, nnode);
     for (int j = 1; j <= ORC_NEN; j++) {
       int no = IX(j, e);
       if (no == 0 || no > nnode) continue;
-      ip[no] += ORC_NEN - 1;
+      ip[no] += n - 1;
     }
+  }
   for (int i = 1; i <= nnode; i++) ip[i] += ip[i - 1];
   int *noviz = work;
   for (int e = 0; e < numel; e++)

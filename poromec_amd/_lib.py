@@ -72,8 +72,14 @@ def lib():
     L.pm_ctx_destroy.restype = None
     L.pm_mesh_setup.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int,
                                 C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+    L.pm_mesh_setup_nen.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int,
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int64)]
     L.pm_mesh_numbering.argtypes = [C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp,
                                     C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+    L.pm_mesh_numbering_nen.argtypes = [C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int,
+                                        _vp, _vp, _vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int64)]
     L.pm_get_numbering.argtypes = [_vp, _vp, _vp, _vp]
     L.pm_set_materials.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double]
     L.pm_set_nodal_loads.argtypes = [_vp, _vp, _vp]

@@ -13,15 +13,18 @@
 //   dJ/dlogE_e = -sum_n int eps(lambda_n)^T D eps(u_n)
 //   dJ/dlogk_e = +sum_n dt perm int grad lambda_p,n . grad p_n.
 // The forward states are recomputed from checkpoints kept every k steps.
+// The kernels serve the Q1/Q1 element (element type 7) too, with its 8
+// trilinear displacement nodes and the 2 x 2 x 2 rule.
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
 #include "pm_elemlib.h"
+#include "pm_q1.h"
 #include "pm_internal.h"
 
 struct AdjArgs {
-  const int *ix;        // (numel, 20)
+  const int *ix;        // (numel, nen): 20 hexa20/8 nodes or 8 Q1/Q1 vertices
   const double *xv;     // (nnodev, 3)
   const int *eq;        // (nnode, 4)
   const int *mat;       // (numel)
@@ -36,118 +39,155 @@ struct AdjArgs {
   double grav[3];
 };
 
+// Gauss point g of the element's volume rule: 4 x 4 x 4 for hexa20/8, the
+// 2 x 2 x 2 rule (weights 1) for Q1/Q1; (lz, ly, lx) with lx fastest
+template <bool Q1>
+__device__ __forceinline__ void adj_point(int g, double &r, double &s, double &t, double &wg) {
+  if (Q1) {
+    r = c_q1_pg[g & 1];
+    s = c_q1_pg[(g >> 1) & 1];
+    t = c_q1_pg[g >> 2];
+    wg = 1.0;
+  } else {
+    const int lx = g & 3, ly = (g >> 2) & 3, lz = g >> 4;
+    r = c_pg4[lx];
+    s = c_pg4[ly];
+    t = c_pg4[lz];
+    wg = c_wg4[lx] * c_wg4[ly] * c_wg4[lz];
+  }
+}
+
+// global derivatives of displacement node k (20 serendipity or 8 trilinear)
+template <bool Q1>
+__device__ __forceinline__ void adj_ugrad(int k, double r, double s, double t,
+                                          const double xji[3][3], double &gx, double &gy,
+                                          double &gz) {
+  double h, dr, ds, dt;
+  if (Q1) dn8(k, r, s, t, dr, ds, dt);
+  else dn20(k, r, s, t, h, dr, ds, dt);
+  to_global(xji, dr, ds, dt, gx, gy, gz);
+}
+
 // C^T lambda = [Kup lambda_p; -S lambda_p], element by element, colour order.
 // One workgroup of 64 threads per element, one thread per Gauss point.
+template <bool Q1>
 __global__ __launch_bounds__(64) void k_adj_coupling(AdjArgs A) {
+  constexpr int NU = Q1 ? 8 : 20, NP = 3 * NU, NS = NP + 8, NG = Q1 ? 8 : 64;
   __shared__ double s_xv[8][3];
   __shared__ double s_lp[8];
-  __shared__ double s_c[64][PM_NST + 1];
-  __shared__ int s_ld[PM_NST];
+  __shared__ double s_c[NG][NS + 1];
+  __shared__ int s_ld[NS];
   __shared__ double s_mat[PM_PROP];
   const int tid = threadIdx.x;
   const int el = A.elems[blockIdx.x];
-  const int *nodes = A.ix + (size_t)el * 20;
+  const int *nodes = A.ix + (size_t)el * NU;
   if (tid < 24) s_xv[tid / 3][tid % 3] = A.xv[(size_t)nodes[tid / 3] * 3 + tid % 3];
   if (tid < 8) {
     const int q = A.eq[(size_t)nodes[tid] * 4 + 3];
     s_lp[tid] = q >= 0 ? A.lam[q] : 0.0;
   }
   if (tid < PM_PROP) s_mat[tid] = A.e[(size_t)A.mat[el] * PM_PROP + tid];
-  for (int i = tid; i < PM_NST; i += 64)
-    s_ld[i] = i < 60 ? A.eq[(size_t)nodes[i / 3] * 4 + i % 3] : A.eq[(size_t)nodes[i - 60] * 4 + 3];
+  for (int i = tid; i < NS; i += 64)
+    s_ld[i] = i < NP ? A.eq[(size_t)nodes[i / 3] * 4 + i % 3] : A.eq[(size_t)nodes[i - NP] * 4 + 3];
   __syncthreads();
   const ElemMat M = elem_material(s_mat, A.dt, A.gmod);
-  const int g = tid, lx = g & 3, ly = (g >> 2) & 3, lz = g >> 4;
-  const double r = c_pg4[lx], s = c_pg4[ly], t = c_pg4[lz];
-  double xji[3][3];
-  const double det = jacobian(s_xv, r, s, t, xji);
-  const double w = c_wg4[lx] * c_wg4[ly] * c_wg4[lz] * det;
-  double q = 0.0, hp[8];
-  for (int k = 0; k < 8; k++) {
-    hp[k] = n8(k, r, s, t);
-    q += hp[k] * s_lp[k];
+  const int g = tid;
+  if (g < NG) {
+    double r, s, t, wg;
+    adj_point<Q1>(g, r, s, t, wg);
+    double xji[3][3];
+    const double det = jacobian(s_xv, r, s, t, xji);
+    const double w = wg * det;
+    double q = 0.0, hp[8];
+    for (int k = 0; k < 8; k++) {
+      hp[k] = n8(k, r, s, t);
+      q += hp[k] * s_lp[k];
+    }
+    for (int k = 0; k < NU; k++) {
+      double gx, gy, gz;
+      adj_ugrad<Q1>(k, r, s, t, xji, gx, gy, gz);
+      const double f = -M.coef_biot * w * q;
+      s_c[g][3 * k] = f * gx;
+      s_c[g][3 * k + 1] = f * gy;
+      s_c[g][3 * k + 2] = f * gz;
+    }
+    for (int k = 0; k < 8; k++) s_c[g][NP + k] = -M.imod_biot * w * hp[k] * q;
   }
-  for (int k = 0; k < 20; k++) {
-    double h, dr, ds, dt, gx, gy, gz;
-    dn20(k, r, s, t, h, dr, ds, dt);
-    to_global(xji, dr, ds, dt, gx, gy, gz);
-    const double f = -M.coef_biot * w * q;
-    s_c[g][3 * k] = f * gx;
-    s_c[g][3 * k + 1] = f * gy;
-    s_c[g][3 * k + 2] = f * gz;
-  }
-  for (int k = 0; k < 8; k++) s_c[g][60 + k] = -M.imod_biot * w * hp[k] * q;
   __syncthreads();
-  for (int i = tid; i < PM_NST; i += 64) {
+  for (int i = tid; i < NS; i += 64) {
     double v = 0.0;
-    for (int p = 0; p < 64; p++) v += s_c[p][i];
+    for (int p = 0; p < NG; p++) v += s_c[p][i];
     const int row = s_ld[i];
     if (row >= 0) A.out[row] += v;
   }
 }
 
 // per-element gradient contributions of one step
+template <bool Q1>
 __global__ __launch_bounds__(64) void k_adj_grad(AdjArgs A) {
+  constexpr int NU = Q1 ? 8 : 20, NP = 3 * NU, NS = NP + 8, NG = Q1 ? 8 : 64;
   __shared__ double s_xv[8][3];
-  __shared__ double s_l[PM_NST], s_s[PM_NST];
+  __shared__ double s_l[NS], s_s[NS];
   __shared__ double s_mat[PM_PROP];
-  __shared__ double s_g[64][2];
+  __shared__ double s_g[NG][2];
   const int tid = threadIdx.x;
   const int el = blockIdx.x;
-  const int *nodes = A.ix + (size_t)el * 20;
+  const int *nodes = A.ix + (size_t)el * NU;
   if (tid < 24) s_xv[tid / 3][tid % 3] = A.xv[(size_t)nodes[tid / 3] * 3 + tid % 3];
   if (tid < PM_PROP) s_mat[tid] = A.e[(size_t)A.mat[el] * PM_PROP + tid];
-  for (int i = tid; i < PM_NST; i += 64) {
-    const size_t ns = i < 60 ? (size_t)nodes[i / 3] * 4 + i % 3 : (size_t)nodes[i - 60] * 4 + 3;
+  for (int i = tid; i < NS; i += 64) {
+    const size_t ns = i < NP ? (size_t)nodes[i / 3] * 4 + i % 3 : (size_t)nodes[i - NP] * 4 + 3;
     const int q = A.eq[ns];
     s_l[i] = q >= 0 ? A.lam[q] : 0.0;
     s_s[i] = A.s[ns];
   }
   __syncthreads();
   const ElemMat M = elem_material(s_mat, A.dt, A.gmod);
-  const int g = tid, lx = g & 3, ly = (g >> 2) & 3, lz = g >> 4;
-  const double r = c_pg4[lx], s = c_pg4[ly], t = c_pg4[lz];
-  double xji[3][3];
-  const double det = jacobian(s_xv, r, s, t, xji);
-  const double w = c_wg4[lx] * c_wg4[ly] * c_wg4[lz] * det;
-  double el6[6] = {0, 0, 0, 0, 0, 0}, es6[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = 0; k < 20; k++) {
-    double h, dr, ds, dt, gx, gy, gz;
-    dn20(k, r, s, t, h, dr, ds, dt);
-    to_global(xji, dr, ds, dt, gx, gy, gz);
-    const double lu = s_l[3 * k], lv = s_l[3 * k + 1], lw = s_l[3 * k + 2];
-    const double su = s_s[3 * k], sv = s_s[3 * k + 1], sw = s_s[3 * k + 2];
-    el6[0] += gx * lu; el6[1] += gy * lv; el6[2] += gz * lw;
-    el6[3] += gy * lu + gx * lv; el6[4] += gz * lv + gy * lw; el6[5] += gz * lu + gx * lw;
-    es6[0] += gx * su; es6[1] += gy * sv; es6[2] += gz * sw;
-    es6[3] += gy * su + gx * sv; es6[4] += gz * sv + gy * sw; es6[5] += gz * su + gx * sw;
+  const int g = tid;
+  if (g < NG) {
+    double r, s, t, wg;
+    adj_point<Q1>(g, r, s, t, wg);
+    double xji[3][3];
+    const double det = jacobian(s_xv, r, s, t, xji);
+    const double w = wg * det;
+    double el6[6] = {0, 0, 0, 0, 0, 0}, es6[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < NU; k++) {
+      double gx, gy, gz;
+      adj_ugrad<Q1>(k, r, s, t, xji, gx, gy, gz);
+      const double lu = s_l[3 * k], lv = s_l[3 * k + 1], lw = s_l[3 * k + 2];
+      const double su = s_s[3 * k], sv = s_s[3 * k + 1], sw = s_s[3 * k + 2];
+      el6[0] += gx * lu; el6[1] += gy * lv; el6[2] += gz * lw;
+      el6[3] += gy * lu + gx * lv; el6[4] += gz * lv + gy * lw; el6[5] += gz * lu + gx * lw;
+      es6[0] += gx * su; es6[1] += gy * sv; es6[2] += gz * sw;
+      es6[3] += gy * su + gx * sv; es6[4] += gz * sv + gy * sw; es6[5] += gz * su + gx * sw;
+    }
+    double tx[6];
+    tx[0] = (es6[0] + M.b * es6[1] + M.b * es6[2]) * M.a;
+    tx[1] = (M.b * es6[0] + es6[1] + M.b * es6[2]) * M.a;
+    tx[2] = (M.b * es6[0] + M.b * es6[1] + es6[2]) * M.a;
+    tx[3] = es6[3] * M.c * M.a;
+    tx[4] = es6[4] * M.c * M.a;
+    tx[5] = es6[5] * M.c * M.a;
+    double eDe = 0.0;
+    for (int i = 0; i < 6; i++) eDe += el6[i] * tx[i];
+    double glx = 0, gly = 0, glz = 0, gsx = 0, gsy = 0, gsz = 0;
+    for (int k = 0; k < 8; k++) {
+      double dr, ds, dt, gx, gy, gz;
+      dn8(k, r, s, t, dr, ds, dt);
+      to_global(xji, dr, ds, dt, gx, gy, gz);
+      glx += gx * s_l[NP + k]; gly += gy * s_l[NP + k]; glz += gz * s_l[NP + k];
+      gsx += gx * s_s[NP + k]; gsy += gy * s_s[NP + k]; gsz += gz * s_s[NP + k];
+    }
+    // d r_p / dlog k = -Lk p + G: the flux term and the gravity flux G
+    // (wt1 grad N . g in the isw 4 residual) both scale with the permeability
+    s_g[g][0] = -w * eDe;
+    s_g[g][1] = w * M.dt_perm * (glx * gsx + gly * gsy + glz * gsz) -
+                w * M.dt_fluid_perm * (glx * A.grav[0] + gly * A.grav[1] + glz * A.grav[2]);
   }
-  double tx[6];
-  tx[0] = (es6[0] + M.b * es6[1] + M.b * es6[2]) * M.a;
-  tx[1] = (M.b * es6[0] + es6[1] + M.b * es6[2]) * M.a;
-  tx[2] = (M.b * es6[0] + M.b * es6[1] + es6[2]) * M.a;
-  tx[3] = es6[3] * M.c * M.a;
-  tx[4] = es6[4] * M.c * M.a;
-  tx[5] = es6[5] * M.c * M.a;
-  double eDe = 0.0;
-  for (int i = 0; i < 6; i++) eDe += el6[i] * tx[i];
-  double glx = 0, gly = 0, glz = 0, gsx = 0, gsy = 0, gsz = 0;
-  for (int k = 0; k < 8; k++) {
-    double dr, ds, dt, gx, gy, gz;
-    dn8(k, r, s, t, dr, ds, dt);
-    to_global(xji, dr, ds, dt, gx, gy, gz);
-    glx += gx * s_l[60 + k]; gly += gy * s_l[60 + k]; glz += gz * s_l[60 + k];
-    gsx += gx * s_s[60 + k]; gsy += gy * s_s[60 + k]; gsz += gz * s_s[60 + k];
-  }
-  // d r_p / dlog k = -Lk p + G: the flux term and the gravity flux G
-  // (wt1 grad N . g in the isw 4 residual) both scale with the permeability
-  s_g[g][0] = -w * eDe;
-  s_g[g][1] = w * M.dt_perm * (glx * gsx + gly * gsy + glz * gsz) -
-              w * M.dt_fluid_perm * (glx * A.grav[0] + gly * A.grav[1] + glz * A.grav[2]);
   __syncthreads();
   if (tid == 0) {
     double a = 0.0, b = 0.0;
-    for (int p = 0; p < 64; p++) {
+    for (int p = 0; p < NG; p++) {
       a += s_g[p][0];
       b += s_g[p][1];
     }
@@ -201,6 +241,7 @@ int pm_adjoint_dev(pm_ctx *c, int nsteps, double dt, const pm_solver_cfg *cfg, i
     return PM_ERR_ARG;
   if (pm_batch_nbox(c) > 0) return PM_ERR_UNSUPPORTED;
   const int nn = c->mesh.nnode, neq = c->mesh.neq, numel = c->mesh.numel;
+  const bool q1 = c->mesh.nen == 8;
   std::vector<int> seq(nsens);
   for (int i = 0; i < nsens; i++) {
     const int d = sensor_dof[i];
@@ -310,8 +351,9 @@ int pm_adjoint_dev(pm_ctx *c, int nsteps, double dt, const pm_solver_cfg *cfg, i
         for (int k = 0; k < ncolour; k++) {
           A.elems = c->d_colour_elems + c->colour_ptr[k];
           A.nelem = c->colour_ptr[k + 1] - c->colour_ptr[k];
-          if (A.nelem > 0)
-            hipLaunchKernelGGL(k_adj_coupling, dim3(A.nelem), dim3(64), 0, c->stream, A);
+          if (A.nelem == 0) continue;
+          if (q1) hipLaunchKernelGGL(k_adj_coupling<true>, dim3(A.nelem), dim3(64), 0, c->stream, A);
+          else hipLaunchKernelGGL(k_adj_coupling<false>, dim3(A.nelem), dim3(64), 0, c->stream, A);
         }
         if (nsens > 0)
           hipLaunchKernelGGL(k_sensor_rhs, dim3(gs), dim3(256), 0, c->stream, nsens, d_seq,
@@ -325,7 +367,8 @@ int pm_adjoint_dev(pm_ctx *c, int nsteps, double dt, const pm_solver_cfg *cfg, i
         A.s = seg + (size_t)(n - n0) * nu;
         A.gE = gE;
         A.gk = gk;
-        hipLaunchKernelGGL(k_adj_grad, dim3(numel), dim3(64), 0, c->stream, A);
+        if (q1) hipLaunchKernelGGL(k_adj_grad<true>, dim3(numel), dim3(64), 0, c->stream, A);
+        else hipLaunchKernelGGL(k_adj_grad<false>, dim3(numel), dim3(64), 0, c->stream, A);
         std::swap(lam, lam_next);
       }
     }

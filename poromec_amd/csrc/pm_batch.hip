@@ -568,7 +568,7 @@ int pm_batch_define_dev(pm_ctx *c, int nbox, const int *elem_off) {
   for (int b = 0; b < nbox; b++) {
     if (elem_off[b + 1] <= elem_off[b]) return PM_ERR_ARG;
     for (int e = elem_off[b]; e < elem_off[b + 1]; e++)
-      for (int k = 0; k < 20; k++) {
+      for (int k = 0; k < m.nen; k++) {
         const int n = m.ix20[(size_t)e * 21 + k] - 1;
         const int nd = m.fnno[n] == 1 ? 4 : 3;
         for (int d = 0; d < nd; d++) {

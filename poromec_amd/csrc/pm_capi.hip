@@ -363,22 +363,25 @@ static int ensure_state(pm_ctx *c) {
   return PM_OK;
 }
 
-extern "C" int pm_mesh_setup(pm_ctx *c, int numel, int nnodev, const int *ix, const double *x,
-                             const int *idv, int reorder, int *nnode, int *neq, int64_t *nad) {
+static int mesh_setup(pm_ctx *c, int nen, int numel, int nnodev, const int *ix, const double *x,
+                      const int *idv, int reorder, int *nnode, int *neq, int64_t *nad) {
   if (!c || numel <= 0 || nnodev <= 0 || !ix || !x || !idv) return PM_ERR_ARG;
+  if (nen != 8 && nen != 20) return PM_ERR_ARG;
   PM_CHECK_HIP(hipSetDevice(c->device));
   PmHostMesh &m = c->mesh;
   m = PmHostMesh();
+  c->have_mesh = false;
   FREE(c->d_tf_ptr);  // per-mesh maps are rebuilt on first use
   FREE(c->d_tf_src);
   FREE(c->d_tf_out);
   c->have_ordered_maps = false;
   m.numel = numel;
   m.nnodev = nnodev;
-  PM_TRY(pm_host_promote(m, ix));
+  if (nen == 20) PM_TRY(pm_host_promote(m, ix));
+  else PM_TRY(pm_host_plain_hexa8(m, ix));
   m.idflags.assign((size_t)m.nnode * 4, 0);
   for (size_t i = 0; i < (size_t)nnodev * 4; i++) m.idflags[i] = idv[i];
-  pm_host_bound_quad(m);
+  if (nen == 20) pm_host_bound_quad(m);
   pm_host_graph(m);
   pm_host_rcm(m, reorder);
   pm_host_numeq(m);
@@ -387,10 +390,11 @@ extern "C" int pm_mesh_setup(pm_ctx *c, int numel, int nnodev, const int *ix, co
   PM_TRY(pm_host_pattern(m, p));
   m.nad = pm_host_count_lower(p);
   PM_TRY(pm_upload_pattern(c, p));
-  // device mesh: 0-based nodes, equations (-1 = none), pressure flags, colours
-  std::vector<int> ix0((size_t)numel * 20), eq0((size_t)m.nnode * 4, -1), mat0((size_t)numel);
+  // device mesh: 0-based nodes (nen per element), equations (-1 = none),
+  // pressure flags, colours
+  std::vector<int> ix0((size_t)numel * nen), eq0((size_t)m.nnode * 4, -1), mat0((size_t)numel);
   for (int e = 0; e < numel; e++) {
-    for (int k = 0; k < 20; k++) ix0[(size_t)e * 20 + k] = m.ix20[(size_t)e * 21 + k] - 1;
+    for (int k = 0; k < nen; k++) ix0[(size_t)e * nen + k] = m.ix20[(size_t)e * 21 + k] - 1;
     mat0[e] = m.ix20[(size_t)e * 21 + 20] - 1;
   }
   std::vector<unsigned char> isp((size_t)m.neq, 0);
@@ -427,18 +431,30 @@ extern "C" int pm_mesh_setup(pm_ctx *c, int numel, int nnodev, const int *ix, co
   return PM_OK;
 }
 
-extern "C" int pm_mesh_numbering(int numel, int nnodev, const int *ix, const int *idv,
-                                 int reorder, int nnode_max, int *ix20, int *id, int *perm,
-                                 int *nnode, int *neq, int64_t *nad) {
-  if (numel <= 0 || nnodev <= 0 || !ix || !idv) return PM_ERR_ARG;
+extern "C" int pm_mesh_setup(pm_ctx *c, int numel, int nnodev, const int *ix, const double *x,
+                             const int *idv, int reorder, int *nnode, int *neq, int64_t *nad) {
+  return mesh_setup(c, 20, numel, nnodev, ix, x, idv, reorder, nnode, neq, nad);
+}
+
+extern "C" int pm_mesh_setup_nen(pm_ctx *c, int nen, int numel, int nnodev, const int *ix,
+                                 const double *x, const int *idv, int reorder, int *nnode,
+                                 int *neq, int64_t *nad) {
+  return mesh_setup(c, nen, numel, nnodev, ix, x, idv, reorder, nnode, neq, nad);
+}
+
+static int mesh_numbering(int nen, int numel, int nnodev, const int *ix, const int *idv,
+                          int reorder, int nnode_max, int *ix20, int *id, int *perm, int *nnode,
+                          int *neq, int64_t *nad) {
+  if (numel <= 0 || nnodev <= 0 || !ix || !idv || (nen != 8 && nen != 20)) return PM_ERR_ARG;
   PmHostMesh m;
   m.numel = numel;
   m.nnodev = nnodev;
-  PM_TRY(pm_host_promote(m, ix));
+  if (nen == 20) PM_TRY(pm_host_promote(m, ix));
+  else PM_TRY(pm_host_plain_hexa8(m, ix));
   if (m.nnode > nnode_max) return PM_ERR_ARG;
   m.idflags.assign((size_t)m.nnode * 4, 0);
   for (size_t i = 0; i < (size_t)nnodev * 4; i++) m.idflags[i] = idv[i];
-  pm_host_bound_quad(m);
+  if (nen == 20) pm_host_bound_quad(m);
   pm_host_graph(m);
   pm_host_rcm(m, reorder);
   pm_host_numeq(m);
@@ -451,6 +467,20 @@ extern "C" int pm_mesh_numbering(int numel, int nnodev, const int *ix, const int
   if (neq) *neq = m.neq;
   if (nad) *nad = pm_host_count_lower(p);
   return PM_OK;
+}
+
+extern "C" int pm_mesh_numbering(int numel, int nnodev, const int *ix, const int *idv,
+                                 int reorder, int nnode_max, int *ix20, int *id, int *perm,
+                                 int *nnode, int *neq, int64_t *nad) {
+  return mesh_numbering(20, numel, nnodev, ix, idv, reorder, nnode_max, ix20, id, perm, nnode,
+                        neq, nad);
+}
+
+extern "C" int pm_mesh_numbering_nen(int nen, int numel, int nnodev, const int *ix,
+                                     const int *idv, int reorder, int nnode_max, int *ix20,
+                                     int *id, int *perm, int *nnode, int *neq, int64_t *nad) {
+  return mesh_numbering(nen, numel, nnodev, ix, idv, reorder, nnode_max, ix20, id, perm, nnode,
+                        neq, nad);
 }
 
 extern "C" int pm_get_numbering(pm_ctx *c, int *ix20, int *id, int *perm) {
@@ -815,6 +845,11 @@ extern "C" int pm_set_vprop(pm_ctx *c, const int *ie, const int *vprop) {
   int f[4] = {0, 0, 0, 0};
   for (int i = 0; i < 4 && vprop; i++) f[i] = vprop[i] != 0;
   if (f[1] || f[2] || f[3]) f[0] = 1;  // read_constitutive_equation, Rdata.f:2630-2633
+  if (c->mesh.nen == 8) {  // a maxno-8 mesh: every material is the Q1/Q1 element
+    for (int ma = 0; ma < numat; ma++)
+      if (ie[ma] != 7) return PM_ERR_UNSUPPORTED;
+    return f[0] ? PM_ERR_UNSUPPORTED : PM_OK;
+  }
   bool any18 = false;
   for (int ma = 0; ma < numat; ma++) {
     if (ie[ma] != 17 && ie[ma] != 18) return PM_ERR_UNSUPPORTED;

@@ -20,39 +20,8 @@
 
 #include <algorithm>
 
+#include "pm_elemargs.h"
 #include "pm_elemlib.h"
-
-struct ElemArgs {
-  const int *ix;       // (numel, 20)
-  const double *xv;    // (nnodev, 3)
-  const int *eq;       // (nnode, 4)
-  const int *mat;      // (numel)
-  const double *e;     // (numat, 15)
-  const int *eload;    // (numel, 14)
-  const int *load;     // (200, 3)
-  const double *fload; // (200, 5, 1000)
-  const double *u;     // (nnode, 4)
-  const double *dp;    // (nnode)
-  double *b;           // right-hand side
-  int *err;            // element error flag
-  double dt, t, gmod;
-  double grav[3];
-  // matrix
-  const int *eq_block, *fr;
-  const int64_t *colptr, *valptr;
-  const int *ncol;
-  double *val;
-  int lhs, rhs;
-  const int *elems;    // elements of this colour
-  int nelem;
-  // reference summation order: element arrays are stored, not scattered
-  double *ke;          // (numel, 68, 68) element matrices, or null
-  double *pe;          // (numel, 68) element vectors, or null
-  // variable properties: element type per material (null: all 17) and the
-  // 7 properties of each element's 64 integration points (elmt18_pm)
-  const int *ie;
-  const double *vp;    // (numel, 64, 7)
-};
 
 __device__ __forceinline__ bool is_elmt18(const ElemArgs &A, int el) {
   return A.ie && A.ie[A.mat[el]] == 18;
@@ -287,15 +256,6 @@ __global__ __launch_bounds__(PM_WG) void k_elmt_isw2(ElemArgs A) {
 // ---------------------------------------------------------------------------
 // isw = 4: volume terms at t+dt from the state at t, face loads
 // ---------------------------------------------------------------------------
-__device__ double dev_interpol(const double *xs, const double *ys, double xi, int n) {
-  if (xi <= xs[0]) return ys[0];
-  if (xi >= xs[n - 1]) return ys[n - 1];
-  for (int i = 0; i < n - 1; i++)
-    if (xi >= xs[i] && xi <= xs[i + 1]) return ys[i] + (ys[i + 1] - ys[i]) * (xi - xs[i]) / (xs[i + 1] - xs[i]);
-  return 0.0;
-}
-#define DFLOAD(i, term, nc) (A.fload + (size_t)((nc) - 1) * PM_NTERMS * PM_NPAR + (size_t)((term) - 1) * PM_NPAR + (i) - 1)
-
 __device__ void rot_face(double xf[4][3]) {
   double v1x = xf[1][0] - xf[0][0], v1y = xf[1][1] - xf[0][1], v1z = xf[1][2] - xf[0][2];
   double v2x = xf[2][0] - xf[0][0], v2y = xf[2][1] - xf[0][1], v2z = xf[2][2] - xf[0][2];
@@ -760,7 +720,10 @@ int pm_pform_dev(pm_ctx *c, int isw, int lhs, int rhs, double dt, double t, doub
     c->A.have32 = false;
   }
   ElemArgs A = elem_args(c, dt, t, bvec, lhs, rhs);
-  if (c->ordered) {
+  const bool q1 = c->mesh.nen == 8;
+  if (q1 && isw != 2 && isw != 4) return PM_ERR_ARG;
+  if (c->ordered) {  // the Q1/Q1 element has no reference order to follow
+    if (q1) return PM_ERR_UNSUPPORTED;
     PM_TRY(ensure_ordered(c));
     if (isw == 2 && lhs) A.ke = c->d_ke;
     if (rhs) A.pe = c->d_pe;
@@ -770,7 +733,9 @@ int pm_pform_dev(pm_ctx *c, int isw, int lhs, int rhs, double dt, double t, doub
     A.elems = c->d_colour_elems + c->colour_ptr[k];
     A.nelem = c->colour_ptr[k + 1] - c->colour_ptr[k];
     if (A.nelem == 0) continue;
-    if (isw == 2)
+    if (q1)
+      PM_TRY(pm_q1_elements(A, isw, c->stream));
+    else if (isw == 2)
       hipLaunchKernelGGL(k_elmt_isw2, dim3(A.nelem), dim3(PM_WG), 0, c->stream, A);
     else if (isw == 4)
       hipLaunchKernelGGL(k_elmt_isw4, dim3(A.nelem), dim3(PM_WG), 0, c->stream, A);
@@ -812,7 +777,9 @@ int pm_porosity_dev(pm_ctx *c, double dt) {
   PM_CHECK_HIP(hipMemsetAsync(c->d_elem_err, 0, sizeof(int), c->stream));
   ElemArgs A = elem_args(c, dt, c->t, nullptr, 0, 0);
   double *pv = nullptr;
+  const bool q1 = c->mesh.nen == 8;
   if (c->ordered) {
+    if (q1) return PM_ERR_UNSUPPORTED;
     PM_TRY(ensure_ordered(c));
     pv = c->d_pv;
   }
@@ -821,6 +788,10 @@ int pm_porosity_dev(pm_ctx *c, double dt) {
     A.elems = c->d_colour_elems + c->colour_ptr[k];
     A.nelem = c->colour_ptr[k + 1] - c->colour_ptr[k];
     if (A.nelem == 0) continue;
+    if (q1) {
+      PM_TRY(pm_q1_porosity(A, c->d_dporo, c->stream));
+      continue;
+    }
     int g = (A.nelem + PM_WG / 8 - 1) / (PM_WG / 8);
     hipLaunchKernelGGL(k_elmt_isw7, dim3(g), dim3(PM_WG), 0, c->stream, A, c->d_dporo, pv);
   }

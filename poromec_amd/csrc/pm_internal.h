@@ -40,8 +40,9 @@ void pm_set_last_error(const char *msg, const char *file, int line);
 // ---------------------------------------------------------------------------
 struct PmHostMesh {
   int numel = 0, nnodev = 0, nnode = 0, neq = 0;
+  int nen = 20;             // 20: hexa20/8 (promoted mesh); 8: Q1/Q1, nodes = vertices
   int64_t nad = 0;          // strict-lower count (reference CSRC size)
-  std::vector<int> ix20;    // (21, numel) 1-based, reference layout
+  std::vector<int> ix20;    // (21, numel) 1-based, reference layout (nen 8: columns 9..20 zero)
   std::vector<int> idflags; // (4, nnode) restraint flags
   std::vector<int> id;      // (4, nnode) reference equation numbers (1-based, 0 none)
   std::vector<int> perm;    // (nnode) RCM order, 1-based node ids
@@ -69,6 +70,9 @@ struct PmHostPattern {
 };
 
 int pm_host_promote(PmHostMesh &m, const int *ix8);
+// a maxno-8 mesh as read: ix9 (9, numel) = 8 vertices + material, no
+// promotion, every node a vertex (the Q1/Q1 element)
+int pm_host_plain_hexa8(PmHostMesh &m, const int *ix9);
 void pm_host_bound_quad(PmHostMesh &m);
 void pm_host_graph(PmHostMesh &m);
 void pm_host_rcm(PmHostMesh &m, int reorder);

@@ -93,6 +93,23 @@ int pm_host_promote(PmHostMesh &m, const int *ix8) {
   return PM_OK;
 }
 
+int pm_host_plain_hexa8(PmHostMesh &m, const int *ix9) {
+  const int numel = m.numel, nnodev = m.nnodev;
+  m.nen = 8;
+  m.ix20.assign((size_t)numel * 21, 0);
+  for (int e = 0; e < numel; e++) {
+    for (int k = 0; k < 8; k++) {
+      const int v = ix9[(size_t)e * 9 + k];
+      if (v < 1 || v > nnodev) return PM_ERR_ARG;
+      m.ix20[(size_t)e * 21 + k] = v;
+    }
+    m.ix20[(size_t)e * 21 + 20] = ix9[(size_t)e * 9 + 8];
+  }
+  m.nnode = nnodev;
+  m.fnno.assign((size_t)nnodev, 1);
+  return PM_OK;
+}
+
 void pm_host_bound_quad(PmHostMesh &m) {
   int *id = m.idflags.data();
   for (int i = 0; i < m.numel; i++) {
@@ -108,24 +125,24 @@ void pm_host_bound_quad(PmHostMesh &m) {
 // node graph: neighbours of each node = all other nodes of its elements,
 // sorted ascending, without repeats
 void pm_host_graph(PmHostMesh &m) {
-  const int numel = m.numel, nnode = m.nnode;
-  // node -> incident elements (all 20 nodes)
+  const int numel = m.numel, nnode = m.nnode, nen = m.nen;
+  // node -> incident elements (all nen nodes)
   std::vector<int64_t> eptr((size_t)nnode + 1, 0);
   for (int e = 0; e < numel; e++)
-    for (int k = 0; k < 20; k++) eptr[m.ix20[(size_t)e * 21 + k]]++;
+    for (int k = 0; k < nen; k++) eptr[m.ix20[(size_t)e * 21 + k]]++;
   for (int n = 0; n < nnode; n++) eptr[n + 1] += eptr[n];
   std::vector<int> einc((size_t)eptr[nnode]);
   {
     std::vector<int64_t> fill(eptr.begin(), eptr.end() - 1);
     for (int e = 0; e < numel; e++)
-      for (int k = 0; k < 20; k++) einc[fill[m.ix20[(size_t)e * 21 + k] - 1]++] = e;
+      for (int k = 0; k < nen; k++) einc[fill[m.ix20[(size_t)e * 21 + k] - 1]++] = e;
   }
   std::vector<int> deg((size_t)nnode, 0);
   auto neighbours = [&](int n, std::vector<int> &buf) {
     buf.clear();
     for (int64_t k = eptr[n]; k < eptr[n + 1]; k++) {
       const int *row = m.ix20.data() + (size_t)einc[k] * 21;
-      for (int j = 0; j < 20; j++)
+      for (int j = 0; j < nen; j++)
         if (row[j] != n + 1) buf.push_back(row[j]);
     }
     std::sort(buf.begin(), buf.end());
@@ -501,6 +518,7 @@ static inline int64_t block_pos(const PmHostPattern &p, int r, int c) {
 }
 
 int pm_host_ordered_maps(const PmHostMesh &m, const PmHostPattern &p, PmOrderedMaps &o) {
+  if (m.nen != 20) return PM_ERR_UNSUPPORTED;  // hexa20/8 element arrays only
   const int numel = m.numel, nst = PM_NST;
   const int64_t nnz = p.valptr[p.nrn];
   std::vector<int> eqs((size_t)numel * nst);

@@ -14,12 +14,13 @@
 // bit for bit, given the same u and dp).
 #include "pm_device.h"
 #include "pm_elemlib.h"
+#include "pm_q1.h"
 
 struct TformArgs {
   int n0, n1;          // nodes of this launch
   const int64_t *ptr;  // (nnode + 1) (element, corner) pairs of each node
   const int *src;      // element * 8 + corner, ascending element
-  const int *ix;       // (numel, 20) 0-based
+  const int *ix;       // (numel, nen) 0-based
   const double *xv;    // (nnodev, 3)
   const int *mat;      // (numel)
   const double *e;     // (numat, PM_PROP)
@@ -106,7 +107,48 @@ __device__ void corner_elmt18(const TformArgs &A, int el, int v, const int *node
   for (int c = 0; c < 3; c++) fl[c] = extrapol_vertex(F[12 + c], v);
 }
 
-template <bool V18>
+// Q1/Q1 element (element type 7) at vertex v: the CPU statement's isw 3
+// -- D eps - alpha dp and + alpha p, perm (rho_f g - grad p) -- from the
+// trilinear fields
+__device__ void corner_q1(const TformArgs &A, int v, const int *nodes, const double (*xv)[3],
+                          const double *e, double *st, double *sb, double *fl) {
+  const double fluid_d = e[6] * 1.0e-06;
+  const double perm = e[2] / (fluid_d * A.gmod);
+  const double coef_biot = e[4], ps = e[1];
+  const double a1 = 1.0 - ps, a2 = 1.0 - 2.0 * ps, a3 = 1.0 + ps;
+  const double a = (e[0] * a1) / (a3 * a2), b = ps / a1, c = 0.5 * (a2 / a1);
+  double h[8], hr[8], hs[8], ht[8], ji[3][3], hx[8], hy[8], hz[8], u[24], eps[6];
+  q1_shape(c_q1_rn[v], c_q1_sn[v], c_q1_tn[v], h, hr, hs, ht);
+  const double det = q1_jacobian(xv, hr, hs, ht, ji);
+  if (det <= 1.0e-14) atomicExch(A.err, PM_ERR_JACOBIAN);
+  for (int k = 0; k < 8; k++) {
+    q1_grad(ji, hr[k], hs[k], ht[k], hx[k], hy[k], hz[k]);
+    for (int d = 0; d < 3; d++) u[3 * k + d] = A.u[(size_t)nodes[k] * 4 + d];
+  }
+  q1_strain(hx, hy, hz, u, eps);
+  q1_stress(a, b, c, eps, st);
+  const int nv = nodes[v];
+  const double dpm = coef_biot * A.dp[nv], pm = coef_biot * A.u[(size_t)nv * 4 + 3];
+  st[0] = st[0] - dpm;
+  st[1] = st[1] - dpm;
+  st[2] = st[2] - dpm;
+  for (int k = 0; k < 6; k++) sb[k] = st[k] + (k < 3 ? pm : 0.0);
+  fl[0] = fluid_d * A.grav[0];
+  fl[1] = fluid_d * A.grav[1];
+  fl[2] = fluid_d * A.grav[2];
+  for (int k = 0; k < 8; k++) {
+    const double pk = A.u[(size_t)nodes[k] * 4 + 3];
+    fl[0] = fl[0] - hx[k] * pk;
+    fl[1] = fl[1] - hy[k] * pk;
+    fl[2] = fl[2] - hz[k] * pk;
+  }
+  fl[0] = fl[0] * perm;
+  fl[1] = fl[1] * perm;
+  fl[2] = fl[2] * perm;
+}
+
+// ET: 17 (elmt17_pm), 18 (elmt17_pm or elmt18_pm by material), 7 (Q1/Q1)
+template <int ET>
 __global__ __launch_bounds__(256) void k_tform(TformArgs A) {
   // each XCD takes a contiguous range of nodes, so the elements that
   // neighbouring nodes share are gathered through one L2
@@ -117,14 +159,15 @@ __global__ __launch_bounds__(256) void k_tform(TformArgs A) {
   for (int64_t q = p0; q < p1; q++) {
     const int code = A.src[q];
     const int el = code >> 3, v = code & 7;
-    const int *nodes = A.ix + (size_t)el * 20;
+    const int *nodes = A.ix + (size_t)el * (ET == 7 ? 8 : 20);
     double xv[8][3];
     for (int k = 0; k < 8; k++)
       for (int d = 0; d < 3; d++) xv[k][d] = A.xv[(size_t)nodes[k] * 3 + d];
     const double *e = A.e + (size_t)A.mat[el] * PM_PROP;
-    if (V18 && A.ie[A.mat[el]] == 18) {
+    if (ET == 7 || (ET == 18 && A.ie[A.mat[el]] == 18)) {
       double st[6], sb[6], fl[3];
-      corner_elmt18(A, el, v, nodes, xv, e, st, sb, fl);
+      if (ET == 7) corner_q1(A, v, nodes, xv, e, st, sb, fl);
+      else corner_elmt18(A, el, v, nodes, xv, e, st, sb, fl);
       for (int j = 0; j < 6; j++) {
         t[j] = t[j] + st[j];
         tb[j] = tb[j] + sb[j];
@@ -287,12 +330,13 @@ int pm_tform_dev(pm_ctx *c, double *tx, double *txe, double *txb, double *flux) 
   A.n0 = 0;
   A.n1 = nv;
   const int gv = (((nv + 255) / 256) + 7) & ~7;
-  if (v18) hipLaunchKernelGGL(k_tform<true>, dim3(gv), dim3(256), 0, c->stream, A);
-  else hipLaunchKernelGGL(k_tform<false>, dim3(gv), dim3(256), 0, c->stream, A);
+  if (c->mesh.nen == 8) hipLaunchKernelGGL(k_tform<7>, dim3(gv), dim3(256), 0, c->stream, A);
+  else if (v18) hipLaunchKernelGGL(k_tform<18>, dim3(gv), dim3(256), 0, c->stream, A);
+  else hipLaunchKernelGGL(k_tform<17>, dim3(gv), dim3(256), 0, c->stream, A);
   if (nn > nv) {
     A.n0 = nv;
     A.n1 = nn;
-    hipLaunchKernelGGL(k_tform<false>, dim3((nn - nv + 255) / 256), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(k_tform<17>, dim3((nn - nv + 255) / 256), dim3(256), 0, c->stream, A);
   }
   PM_CHECK_HIP(hipGetLastError());
   int err = 0;

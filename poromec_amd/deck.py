@@ -30,7 +30,10 @@ also read, each where the current reader would reject or misread it:
   reader does (``Rdata.f:3196``);
 * ``loads`` entries of type 39-43 given inline as ``nc type n v1 .. vn``,
   a load constant in time;
-* element type 7 for the hexa20/8 Biot element (17 in the current library);
+* element type 7 for the hexa20/8 Biot element (17 in the current library)
+  in a ``maxno 20`` mesh; a ``maxno 8`` mesh (hexa8 kept as read) is the
+  equal-order Q1/Q1 element, type 7 -- the ``ELMT07_PM`` the reference
+  reserves without implementing it (``Elmlib03.f:22-23``);
 * blocks the current reader does not know (``faceR1`` ... ``end faceR1``):
   their ``end <name>`` line closes that block and not the mesh block.
 
@@ -161,7 +164,7 @@ class MefInput:
     skipped: list = field(default_factory=list)
     element_type: int = 17
     path: str = ""
-    ie: np.ndarray = None          # (numat) element type per material, 17 or 18
+    ie: np.ndarray = None          # (numat) element type per material: 17 or 18, or 7 (maxno 8)
     vprop: tuple = (False, False, False, False)  # conseq: vprop, kozeny_carman, density, hashin_shtrikman
     print_flags: tuple = DEFAULT_FLAGS  # setprint (print_flag 1-13) as the pres macro uses it
     vtk_legacy: bool = False       # config legacy_vtk: .vtk instead of .vtu
@@ -252,13 +255,17 @@ def read_deck(path) -> MefInput:
     m = st["mesh"]
     deck = Deck(0, 0, 0, m["ix"], m["x"], m["idv"], m["e"], m["eload"], m["eloadp"], m["load"],
                 m["fload"], gravity=m["gravity"], gravity_mod=m["gravity_mod"], dt=st["dt"],
-                nsteps=sum(1 for op in program if op[0] == "solv"))
+                nsteps=sum(1 for op in program if op[0] == "solv"), nen=m["nen"])
     dts = [op[1] for op in program if op[0] == "dt"]
     if dts:
         deck.dt = dts[0]
     if cfg.solver == "pcg" and cfg.precond not in (1, 2, 5):
         raise DeckError(f"preconditioner {cfg.precond} is outside this path")
-    ie = np.where(m["ie"] == 7, 17, m["ie"]).astype(np.int32)
+    if m["nen"] == 8:  # maxno 8: every material the Q1/Q1 element, no conseq
+        if (m["ie"] != 7).any() or st["vprop"][0]:
+            raise DeckError("maxno 8: every material must be element type 7 (Q1/Q1), "
+                            "without conseq vprop")
+    ie = np.where(m["ie"] == 7, 7 if m["nen"] == 8 else 17, m["ie"]).astype(np.int32)
     any18 = bool((ie == 18).any())
     if st["vprop"][0] != any18:  # check_element, Rdata.f:3744-3758
         raise DeckError("Erro: Invalid elements!! (elmt18 materials go with conseq vprop, "
@@ -428,13 +435,15 @@ def _read_mesh(src, st):
     if par["ndf"] != NDF or par["dim"] != 3:
         raise DeckError(f"ndf {par['ndf']} dim {par['dim']}: this path is the 3-D u-p system "
                         "(ndf 4, dim 3)")
-    if par["maxno"] != NEN:
-        raise DeckError(f"maxno {par['maxno']}: this path is the hexa20/8 element (maxno 20)")
+    if par["maxno"] not in (8, NEN):
+        raise DeckError(f"maxno {par['maxno']}: this path is the hexa20/8 element (maxno 20) "
+                        "or the Q1/Q1 element (maxno 8)")
     m = dict(ix=np.zeros((numel, NEN + 1), np.int32), x=np.zeros((nnode, 3)),
              idv=np.zeros((nnode, NDF), np.int32), e=np.zeros((numat, PROP)),
              ie=np.zeros(numat, np.int32), eload=np.zeros((numel, 7), np.int32),
              eloadp=np.zeros((numel, 7), np.int32), u0=np.zeros((nnode, NDF)),
-             gravity=st["gravity"].copy(), gravity_mod=st["gravity_mod"], etype=17)
+             gravity=st["gravity"].copy(), gravity_mod=st["gravity_mod"], etype=17,
+             nen=par["maxno"])
     m["load"], m["fload"] = empty_load_table()
     have_el = have_x = False
     while True:
@@ -781,7 +790,8 @@ def write_deck(path, deck: Deck, cfg: SolverConfig | None = None, u0=None, reord
     every material or a list per material; ``vprop`` the conseq flags
     (vprop, kozeny_carman, density, hashin_shtrikman).  The program is
     ``dt``, ``pnup``, then ``nsteps`` (default ``deck.nsteps``) times
-    ``solv`` + ``pnup``.  Returns the path."""
+    ``solv`` + ``pnup``.  A deck with ``nen = 8`` is written with ``maxno
+    8`` and element type 7 (Q1/Q1).  Returns the path."""
     cfg = cfg or SolverConfig()
     nsteps = deck.nsteps if nsteps is None else nsteps
     path = os.fspath(path)
@@ -809,9 +819,9 @@ def write_deck(path, deck: Deck, cfg: SolverConfig | None = None, u0=None, reord
         L.append("conseq " + side("conseq", "".join(f"{n}\n" for n, f in zip(names, vprop) if f)
                                   + "end\n"))
     L.append("mesh")
-    L.append(f"nnode {nnode} numel {numel} numat {numat} maxno {NEN} ndf {NDF} dim 3")
+    L.append(f"nnode {nnode} numel {numel} numat {numat} maxno {deck.nen} ndf {NDF} dim 3")
     L.append("materials")
-    types = np.broadcast_to(np.asarray(element_type), (numat,))
+    types = np.broadcast_to(np.asarray(7 if deck.nen == 8 else element_type), (numat,))
     for ma in range(numat):
         row = deck.e[ma]
         last = max(7, int(np.nonzero(row)[0].max()) + 1 if row.any() else 7)

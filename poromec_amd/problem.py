@@ -19,7 +19,12 @@ the same arrays for the box problems the benchmarks use:
   carried by ``ie``;
 * BCs: roller walls (ux on the x faces, uy on the y faces), fixed base (uz),
   drained top (p = 0);
-* top load: a type-40 face load (``tload``, ``Pload.f:330-336``) on face 1.
+* top load: a type-40 face load (``tload``, ``Pload.f:330-336``) on face 1;
+* element family: ``nen = 20`` reads the mesh as the reference does with
+  ``maxno 20`` (promoted to the hexa20/8 element ``elmt17_pm``); ``nen = 8``
+  keeps it as read with ``maxno 8``, for the equal-order Q1/Q1 element 7
+  (the ``ELMT07_PM`` the reference reserves, ``Elmlib03.f:22-23``), whose
+  vertex-only dofs give the benchmark configurations' nominal counts.
 """
 from __future__ import annotations
 
@@ -57,6 +62,7 @@ class Deck:
     gravity_mod: float = 10.0
     dt: float = 0.01
     nsteps: int = 20
+    nen: int = 20           # 20: hexa20/8 (elmt17_pm); 8: Q1/Q1 (element type 7)
 
     @property
     def numel(self) -> int:
@@ -169,7 +175,7 @@ def gaussian_field(shape, corr_len: float, rng: np.random.Generator) -> np.ndarr
 
 
 def column_deck(nx=4, ny=4, nz=40, E=10.0, nu=0.3, k=1.0e-10, mu=1.0e-9, load_mpa=1.0e-3,
-                dt=0.01, nsteps=20) -> Deck:
+                dt=0.01, nsteps=20, nen=20) -> Deck:
     """1-D consolidation column: homogeneous material, 1 kPa step load on the
     top, drained top, roller walls, fixed base (one material, ``numat = 1``)."""
     ix, x = box_mesh(nx, ny, nz)
@@ -182,13 +188,14 @@ def column_deck(nx=4, ny=4, nz=40, E=10.0, nu=0.3, k=1.0e-10, mu=1.0e-9, load_mp
     top = np.arange(ix.shape[0]) >= nx * ny * (nz - 1)
     eload[top, 0] = 1
     eloadp = np.zeros_like(eload)
-    return Deck(nx, ny, nz, ix, x, idv, e, eload, eloadp, load, fload, dt=dt, nsteps=nsteps)
+    return Deck(nx, ny, nz, ix, x, idv, e, eload, eloadp, load, fload, dt=dt, nsteps=nsteps,
+                nen=nen)
 
 
 def heterogeneous_box_deck(n, seed=0, sigma_lnk=1.0, mean_lnk=np.log(1.0e-12), corr_len=8.0,
                            E_range=(5.0, 50.0), nu_range=(0.25, 0.35), mu=1.0e-9,
                            load_mpa=1.0e-3, patch_fraction=None, dt=0.1, nsteps=100,
-                           shape=None) -> Deck:
+                           shape=None, nen=20) -> Deck:
     """Box of n^3 (or ``shape``) hex8 cells, one material per element: log-k
     from a seeded Gaussian random field, E and nu uniform per element.  With
     ``patch_fraction`` the top load covers a Bernoulli mask of the top faces."""
@@ -212,12 +219,16 @@ def heterogeneous_box_deck(n, seed=0, sigma_lnk=1.0, mean_lnk=np.log(1.0e-12), c
         top[top_ids[mask]] = True
     eload[top, 0] = 1
     eloadp = np.zeros_like(eload)
-    return Deck(nx, ny, nz, ix, x, idv, e, eload, eloadp, load, fload, dt=dt, nsteps=nsteps)
+    return Deck(nx, ny, nz, ix, x, idv, e, eload, eloadp, load, fload, dt=dt, nsteps=nsteps,
+                nen=nen)
 
 
-def box_counts(nx, ny, nz):
-    """Structural counts of a promoted hexa20/8 box before boundary conditions."""
+def box_counts(nx, ny, nz, nen=20):
+    """Structural counts of a box before boundary conditions: promoted
+    hexa20/8 (nen 20) or Q1/Q1 (nen 8, 4 dofs at every vertex)."""
     nv = (nx + 1) * (ny + 1) * (nz + 1)
+    if nen == 8:
+        return {"numel": nx * ny * nz, "nnodev": nv, "nnode": nv, "dofs": 4 * nv}
     nn = nv + nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz
     return {"numel": nx * ny * nz, "nnodev": nv, "nnode": nn, "dofs": 3 * nn + nv}
 
@@ -232,7 +243,7 @@ def concat_decks(decks):
     node_off = mat_off = 0
     elem_off = [0]
     for d in decks:
-        assert d.dt == first.dt and np.array_equal(d.load, first.load)
+        assert d.dt == first.dt and np.array_equal(d.load, first.load) and d.nen == first.nen
         a = d.ix.copy()
         a[:, :8] += node_off
         a[:, NEN] += mat_off
@@ -247,11 +258,11 @@ def concat_decks(decks):
         elem_off.append(elem_off[-1] + d.numel)
     deck = Deck(0, 0, 0, np.vstack(ix), np.vstack(x), np.vstack(idv), np.vstack(e),
                 np.vstack(el), np.vstack(elp), first.load, first.fload, first.gravity,
-                first.gravity_mod, first.dt, first.nsteps)
+                first.gravity_mod, first.dt, first.nsteps, first.nen)
     return deck, np.array(elem_off, dtype=np.int32)
 
 
-def batch_decks(nbox=256, seed=0, nmin=12, nmax=32, dt=0.1, nsteps=50):
+def batch_decks(nbox=256, seed=0, nmin=12, nmax=32, dt=0.1, nsteps=50, nen=20):
     """Independent boxes of n^3 cells, n ~ U[nmin, nmax] per box, each with its
     own log-normal permeability (sigma ~ U[0.5, 1.5]) and E ~ U[5, 50] MPa."""
     rng = np.random.default_rng(seed)
@@ -259,6 +270,6 @@ def batch_decks(nbox=256, seed=0, nmin=12, nmax=32, dt=0.1, nsteps=50):
     sigmas = rng.uniform(0.5, 1.5, size=nbox)
     seeds = rng.integers(0, 2**31 - 1, size=nbox)
     decks = [heterogeneous_box_deck(int(n), seed=int(sd), sigma_lnk=float(sg), dt=dt,
-                                    nsteps=nsteps, corr_len=min(8.0, n / 2))
+                                    nsteps=nsteps, corr_len=min(8.0, n / 2), nen=nen)
              for n, sg, sd in zip(ns, sigmas, seeds)]
     return decks

@@ -56,19 +56,29 @@ class SolverConfig:
                            int(self.maxnlit_accept))
 
 
+def _setup_ix(deck: Deck):
+    """The connectivity as the mesh entry points take it: (numel, 21) for a
+    maxno-20 deck, (numel, 9) -- 8 vertices + material -- for maxno 8."""
+    if deck.nen == 8:
+        return L.as_i32(np.concatenate([deck.ix[:, :8], deck.ix[:, NEN:]], axis=1))
+    return L.as_i32(deck.ix)
+
+
 def mesh_numbering(deck: Deck, reorder: bool = True):
-    """Promotion, restraints, RCM and numbering on the host (no device)."""
+    """Promotion (maxno 20), restraints, RCM and numbering on the host (no
+    device)."""
     lib = L.lib()
-    ix = L.as_i32(deck.ix)
+    ix = _setup_ix(deck)
     idv = L.as_i32(deck.idv)
-    cap = deck.nnodev + 12 * deck.numel
+    cap = deck.nnodev + (12 * deck.numel if deck.nen == 20 else 0)
     ix20 = np.zeros((deck.numel, NEN + 1), np.int32)
     idm = np.zeros((cap, NDF), np.int32)
     perm = np.zeros(cap, np.int32)
     nnode, neq, nad = C.c_int(), C.c_int(), C.c_int64()
-    L.check(lib.pm_mesh_numbering(deck.numel, deck.nnodev, L.ptr(ix), L.ptr(idv), int(reorder),
-                                  cap, L.ptr(ix20), L.ptr(idm), L.ptr(perm), C.byref(nnode),
-                                  C.byref(neq), C.byref(nad)), "pm_mesh_numbering")
+    L.check(lib.pm_mesh_numbering_nen(deck.nen, deck.numel, deck.nnodev, L.ptr(ix), L.ptr(idv),
+                                      int(reorder), cap, L.ptr(ix20), L.ptr(idm), L.ptr(perm),
+                                      C.byref(nnode), C.byref(neq), C.byref(nad)),
+            "pm_mesh_numbering_nen")
     n = nnode.value
     return dict(ix=ix20, id=idm[:n].copy(), perm=perm[:n].copy(), nnode=n, neq=neq.value,
                 nad=int(nad.value))
@@ -82,13 +92,19 @@ class MefSolver:
         self.deck = deck
         self.ctx = C.c_void_p()
         L.check(self.lib.pm_ctx_create(C.byref(self.ctx), device), "pm_ctx_create")
-        ix = L.as_i32(deck.ix)
         x = L.as_f64(deck.x)
         idv = L.as_i32(deck.idv)
+        ix = _setup_ix(deck)
         nnode, neq, nad = C.c_int(), C.c_int(), C.c_int64()
-        L.check(self.lib.pm_mesh_setup(self.ctx, deck.numel, deck.nnodev, L.ptr(ix), L.ptr(x),
-                                       L.ptr(idv), int(reorder), C.byref(nnode), C.byref(neq),
-                                       C.byref(nad)), "pm_mesh_setup")
+        if deck.nen == 8:  # maxno 8: the Q1/Q1 element on the mesh as read
+            L.check(self.lib.pm_mesh_setup_nen(self.ctx, 8, deck.numel, deck.nnodev, L.ptr(ix),
+                                               L.ptr(x), L.ptr(idv), int(reorder),
+                                               C.byref(nnode), C.byref(neq), C.byref(nad)),
+                    "pm_mesh_setup_nen")
+        else:
+            L.check(self.lib.pm_mesh_setup(self.ctx, deck.numel, deck.nnodev, L.ptr(ix),
+                                           L.ptr(x), L.ptr(idv), int(reorder), C.byref(nnode),
+                                           C.byref(neq), C.byref(nad)), "pm_mesh_setup")
         self.nnode, self.neq, self.nad = nnode.value, neq.value, int(nad.value)
         self.set_materials(deck)
         self.t = 0.0

@@ -96,6 +96,25 @@ def test_write_read_round_trip(tmp_path):
     assert truncate(inp, 2).program == [("dt", d.dt), ("pnup",)] + [("solv",), ("pnup",)] * 2
 
 
+def test_q1_deck_round_trip(tmp_path):
+    """A maxno-8 deck (the Q1/Q1 element, type 7) written and read back:
+    the mesh stays hexa8 (nen 8) and the materials type 7; a maxno-8 deck
+    naming another element type is refused."""
+    d = P.heterogeneous_box_deck(3, seed=2, nen=8)
+    p = write_deck(tmp_path / "q1.dat", d, SolverConfig(stop_crit=1), nsteps=2)
+    text = (tmp_path / "q1.dat").read_text()
+    assert " maxno 8 " in text
+    inp = read_deck(p)
+    assert inp.deck.nen == 8 and np.all(inp.ie == 7)
+    for name in ("ix", "x", "idv", "e", "eload", "eloadp"):
+        assert np.array_equal(getattr(inp.deck, name), getattr(d, name)), name
+    from poromec_amd.solver import mesh_numbering
+    assert mesh_numbering(inp.deck)["nnode"] == d.nnodev
+    (tmp_path / "q1.dat").write_text(text.replace("\n1 7 ", "\n1 17 "))
+    with pytest.raises(DeckError):
+        read_deck(tmp_path / "q1.dat")
+
+
 CURRENT = """config cfg.dat
 solver slv.dat
 gravity grav.dat
