@@ -8,6 +8,9 @@ length 8 cells), E ~ U[5, 50] MPa and nu ~ U[0.25, 0.35] per element, a seeded
 is one backward-Euler time step of the solv macro.  The default n = 87 gives
 8.67M equations, the 8.6M-dof size the metric is quoted at (with the
 reference's element a 128^3-cell box has 27.6M equations; `--n 128` runs it).
+`--element q1` runs the same workload with the equal-order Q1/Q1 hexa8
+element (element type 7, the mesh kept as read), default n = 128: the
+configuration's 128^3-cell mesh at its nominal 8.6M dofs.
 
 The FP32+IR form (FP32 matrix values and Jacobi inside an FP64
 iterative-refinement loop) runs first, then the FP64 form, each from the zero
@@ -39,15 +42,17 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "time-steps/s at 8.6M-dof Biot u-p (FP32+IR vs FP64 @1e-10); SpMV GB/s"
 # (lanes per node block, columns in flight per lane, index prefetch / runs)
-# of the SpMV shapes 0..14
-shape_lanes = (64, 64, 64, 64, 64, 32, 64, 64, 64, 64, 64, 64, 64, 64, 64)
-shape_unr = (1, 2, 3, 4, 5, 4, 3, 4, 5, 1, 2, 1, 2, 5, 1)
-shape_pf = (0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 3, 3, 4, 5)
+# of the SpMV shapes 0..15
+shape_lanes = (64, 64, 64, 64, 64, 32, 64, 64, 64, 64, 64, 64, 64, 64, 64, 32)
+shape_unr = (1, 2, 3, 4, 5, 4, 3, 4, 5, 1, 2, 1, 2, 5, 1, 1)
+shape_pf = (0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 3, 3, 4, 5, 6)
 
 
 def spmv_kernel_name(prec, shape):
     """The SpMV kernel the solver launches at this shape, with its fused dot."""
     vt = "float" if prec == "fp32" else "double"
+    if shape_pf[shape] == 6:
+        return f"k_spmv_ch2<{vt}, 1, false>"
     if shape_pf[shape] == 5:
         return f"k_spmv_cha<{vt}, 1, false>"
     if shape_pf[shape] == 4:
@@ -74,7 +79,7 @@ def spmv_bytes(info, value_bytes, fused_dot=False, shape=None):
     each, first row and row count: 4 B each per block), or for the
     run-per-lane shapes one 32-bit word per run of <= 4 consecutive columns
     (plus chptr: 8 B per block)."""
-    runs = shape is not None and shape_pf[shape] in (2, 3, 5)
+    runs = shape is not None and shape_pf[shape] in (2, 3, 5, 6)
     cols = 4 * info["nruns"] if runs else 4 * info["ncols"]
     meta = (32 if runs else 24) * info["nblocks"]
     return value_bytes * info["nnz"] + cols + meta + (24 if fused_dot else 16) * info["neq"]
@@ -151,7 +156,8 @@ def cpu_baseline(solver, deck, fp64_iters_per_step, fp64_nlit, seed=0):
     fnno = np.zeros(nnode, np.int32)
     fnno[: deck.nnodev] = 1
     st = dict(ix=np.ascontiguousarray(ix20), nnode=nnode, nnodev=deck.nnodev, numel=deck.numel,
-              id=np.ascontiguousarray(idm), fnno=fnno, neq=neq, nad=int(ia[-1] - 1), ia=ia, ja=ja)
+              id=np.ascontiguousarray(idm), fnno=fnno, neq=neq, nad=int(ia[-1] - 1), ia=ia, ja=ja,
+              nen=deck.nen)
     m = O.OracleModel(deck, st)
     m.ad[:] = ad
     mvec = 1.0 / ad  # Jacobi, precond 2 (pre_diag, Precond.f:22-44)
@@ -274,7 +280,7 @@ def adjoint_form(args):
     from poromec_amd.adjoint import MarchMisfit
     from poromec_amd.solver import MefSolver, SolverConfig
 
-    n = args.n
+    n = args.n if args.n is not None else 87
     deck = P.heterogeneous_box_deck(n, seed=args.seed, dt=0.1, nsteps=args.nsteps,
                                     corr_len=min(8.0, n / 2))
     cfg = SolverConfig(tol=1e-13, maxit=200000)
@@ -333,7 +339,7 @@ def tform_form(args):
         dp[deck.nnodev:] = 0.0
         return u, dp
 
-    n = args.n
+    n = args.n if args.n is not None else 87
     deck = P.heterogeneous_box_deck(n, seed=args.seed, dt=0.1, nsteps=1)
     s = MefSolver(deck)
     u, dp = state(deck, s.nnode)
@@ -390,7 +396,7 @@ def vprop_form(args):
         d.e[:, 5], d.e[:, 7] = 2000.0, 0.3
         return d
 
-    n = args.n
+    n = args.n if args.n is not None else 87
     cfg = SolverConfig(tol=1e-10, stop_crit=1)
     out = {"form": "vprop", "n": n}
     for label, ie in (("elmt17", None), ("elmt18", 18)):
@@ -437,7 +443,13 @@ def main():
                          "--only fp64 reports the FP64 form's time steps per second as value")
     ap.add_argument("--live-every", type=int, default=16,
                     help="time every k-th solver SpMV with HIP events")
-    ap.add_argument("--n", type=int, default=87, help="box edge in cells")
+    ap.add_argument("--n", type=int, default=None,
+                    help="box edge in cells (default 87 for hexa20/8, 128 for Q1/Q1: 8.6M dofs "
+                         "either way)")
+    ap.add_argument("--element", choices=["hexa20", "q1"], default="hexa20",
+                    help="hexa20: the reference's hexa20/8 element (maxno 20); q1: the "
+                         "equal-order Q1/Q1 hexa8 element (element type 7, maxno 8), whose "
+                         "vertex-only dofs are the configurations' nominal counts")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--spmv-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -481,15 +493,18 @@ def main():
     from poromec_amd import problem as P
     from poromec_amd.solver import MefSolver, SolverConfig
 
-    n = args.n
+    q1 = args.element == "q1"
+    n = args.n if args.n is not None else (128 if q1 else 87)
     budget = args.budget
 
     def left():
         return budget - (time.perf_counter() - T_START)
 
     t0 = time.perf_counter()
-    deck = P.heterogeneous_box_deck(n, seed=args.seed, dt=0.1, patch_fraction=0.1, nsteps=50)
-    log(f"deck {n}^3: {deck.numel} elements, {deck.nnodev} vertices ({time.perf_counter()-t0:.1f} s)")
+    deck = P.heterogeneous_box_deck(n, seed=args.seed, dt=0.1, patch_fraction=0.1, nsteps=50,
+                                    nen=8 if q1 else 20)
+    log(f"deck {n}^3 ({args.element}): {deck.numel} elements, {deck.nnodev} vertices "
+        f"({time.perf_counter()-t0:.1f} s)")
     t0 = time.perf_counter()
     solver = MefSolver(deck)
     setup_s = time.perf_counter() - t0
@@ -664,11 +679,13 @@ def main():
         "dtype": "f64" if args.only == "fp64" else "f32-spmv/f64-ir",
         "data": "synthetic",
         "config": {
-            "workload": (f"Biot u-p hexa20/8 box {n}^3 cells ({solver.neq} equations"
-                         + (", the metric's 8.6M-dof size" if n == 87 else "")
+            "workload": (f"Biot u-p {'Q1/Q1 hexa8' if q1 else 'hexa20/8'} box {n}^3 cells "
+                         f"({solver.neq} equations"
+                         + (", the metric's 8.6M-dof size" if n == (128 if q1 else 87) else "")
                          + f"), heterogeneous log-k, 10% patch load, dt 0.1 s, SQMR+Jacobi "
                          f"1e-10, {'FP64' if args.only == 'fp64' else 'FP32+IR'}"),
-            "n": n, "numel": deck.numel, "nnode": solver.nnode, "neq": solver.neq,
+            "n": n, "element": args.element, "numel": deck.numel, "nnode": solver.nnode,
+            "neq": solver.neq,
             "nnz": info["nnz"], "dt": deck.dt, "solver": "sqmr", "precond": "diag",
             "tol": 1e-10, "setup_s": round(setup_s, 2),
         },

@@ -121,7 +121,13 @@ int pm_mesh_setup(pm_ctx *ctx, int numel, int nnodev, const int *ix, const doubl
  * it (Elmlib03.f:22-23).  Every node is then a vertex with u, v, w, p; the
  * element is elmt17_pm's weak form with trilinear displacements and the
  * 2 x 2 x 2 Gauss rule.  On such a mesh pm_set_materials takes element-7
- * materials (same e(1..8)), pm_set_vprop accepts only ie = 7 with vprop off,
+ * materials: e(1..8) as for elmt17_pm, and e(9) = beta, the element's
+ * pressure stabilization (0: none; Kpp gains -tau int grad N . grad N, tau =
+ * beta h^2 / (lambda + 2 mu), h the cube root of the element's volume: the
+ * flow-equation perturbation of Aguilar et al., IJNME 75 (2008) 1282-1300,
+ * whose 1-D value beta = 1/4 removes the equal-order pressure oscillation of
+ * a near-undrained step; pm_adjoint refuses stabilized materials);
+ * pm_set_vprop accepts only ie = 7 with vprop off,
  * and the reference summation order (pm_set_ordered) is refused by the
  * element loop (PM_ERR_UNSUPPORTED); pm_get_numbering returns its ix in the
  * (21, numel) layout with columns 9..20 zero. */
@@ -309,12 +315,14 @@ int pm_bench_spmv(pm_ctx *ctx, int use32, int reps, double *ms_per_launch);
  * consecutive columns per lane, 11-12 the runs and 13 the (64, 5) prefetch
  * shape with the workgroups of each XCD sweeping its blocks together, 14
  * the sweeping runs with one run per lane in blocks of <= 64 runs and two in
- * wider blocks.  It keeps the fastest for FP64
+ * wider blocks, 15 the sweeping runs with 32 lanes per block (two blocks per
+ * wavefront; for the Q1/Q1 element's 27-run blocks; the batching form runs
+ * shape 5's kernel for it).  It keeps the fastest for FP64
  * (use32 = 0) or FP32 values, and returns its index and, if ms_per_shape is
  * not NULL, every shape's average launch time.  The batching form tunes its
  * own kernel.  Shapes sum in different orders: results move at the rounding
  * level when the shape changes, and stay fixed until the next change. */
-#define PM_SPMV_SHAPES 15
+#define PM_SPMV_SHAPES 16
 int pm_tune_spmv(pm_ctx *ctx, int use32, int reps, int *best, double *ms_per_shape);
 /* Selects SpMV shape `shape` (0 .. PM_SPMV_SHAPES-1) without timing. */
 int pm_set_spmv_shape(pm_ctx *ctx, int use32, int shape);

@@ -20,7 +20,8 @@
  * dt perm int grad N . grad p on the pressure rows, then the face loads
  * (types 40, 41 tractions and 43 fluxes over the 4-vertex faces); isw 3:
  * vertex stresses and Darcy flux; isw 7: vertex porosity change.  Material
- * constants e(1..8) mean what they mean for elmt17_pm.
+ * constants e(1..8) mean what they mean for elmt17_pm; e(9) is the element's
+ * own: the pressure stabilization coefficient beta (isw 2 below).
  *
  * Local order (nst = 32): u, v, w of vertex i at 3i-2 .. 3i, p of vertex i
  * at 24 + i (i = 1..8, the reference's hexa8 vertex order).  Volume
@@ -182,6 +183,24 @@ int orc_elmt07(const double *e, const int *iqm, const int *iqh, const double *x,
 
   if (isw == 2) {
     memset(s, 0, sizeof(double) * NST7 * NST7);
+    /* pressure stabilization (e(9) = beta, 0: none): tau = beta h^2 / (lambda
+     * + 2 mu), h = V^(1/3), adds -tau int grad N . grad N to Kpp -- the
+     * perturbation tau d/dt (div grad p) of the flow equation, acting on the
+     * step's pressure change as the storage term does (Aguilar, Gaspar,
+     * Lisbona, Rodrigo, IJNME 75 (2008) 1282-1300; beta = 1/4 is their 1-D
+     * value for equal-order linear elements) */
+    double tau = 0.0;
+    if (e[8] != 0.0) {
+      double vol = 0.0;
+      for (int lz = 0; lz < 2; lz++)
+        for (int ly = 0; ly < 2; ly++)
+          for (int lx = 0; lx < 2; lx++) {
+            shape8(pg[lx], pg[ly], pg[lz], h, hx, hy, hz);
+            vol = vol + to_global(x, hx, hy, hz, &bad);
+          }
+      if (bad) return -1;
+      tau = e[8] * cbrt(vol) * cbrt(vol) / a;
+    }
     for (int lz = 0; lz < 2; lz++)
       for (int ly = 0; ly < 2; ly++)
         for (int lx = 0; lx < 2; lx++) {
@@ -221,6 +240,13 @@ int orc_elmt07(const double *e, const int *iqm, const int *iqh, const double *x,
               S7(24 + i, 24 + j) -= w1 * h[i - 1] * h[j - 1] +
                                     w2 * (hx[i - 1] * hx[j - 1] + hy[i - 1] * hy[j - 1] +
                                           hz[i - 1] * hz[j - 1]);
+          if (tau != 0.0) {
+            double w3 = w * tau;
+            for (int i = 1; i <= 8; i++)
+              for (int j = 1; j <= 8; j++)
+                S7(24 + i, 24 + j) -= w3 * (hx[i - 1] * hx[j - 1] + hy[i - 1] * hy[j - 1] +
+                                            hz[i - 1] * hz[j - 1]);
+          }
         }
     for (int i = 25; i <= 32; i++)
       for (int j = 1; j <= 24; j++) S7(j, i) = S7(i, j);

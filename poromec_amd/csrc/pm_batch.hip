@@ -292,7 +292,7 @@ static void bspmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x, doub
                          batch_args(B), A.neq, A.fr, A.nr, A.colptr, A.valptr, A.chptr, A.chunk,
                          val, x, y, w, B->d_S);
       break;
-    default: PM_BSPMV_CASE(5, 32, 4)
+    default: PM_BSPMV_CASE(5, 32, 4)  // shape 15 has no batched form: shape 5's kernel
   }
 #undef PM_BSPMV_CASE
 #undef PM_BSPMV_CH_CASE

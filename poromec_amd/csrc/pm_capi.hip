@@ -932,6 +932,9 @@ extern "C" int pm_adjoint(pm_ctx *c, int nsteps, double dt, const pm_solver_cfg 
                           double *grad_logE, double *grad_logk) {
   if (!c || !cfg) return PM_ERR_ARG;
   if (c->vprop[0]) return PM_ERR_UNSUPPORTED;  // the adjoint is of elmt17_pm's march
+  if (c->mesh.nen == 8)  // Q1/Q1: its pressure stabilization would make C depend on E
+    for (int ma = 0; ma < c->numat; ma++)
+      if (c->h_e[(size_t)ma * PM_PROP + 8] != 0.0) return PM_ERR_UNSUPPORTED;
   PM_CHECK_HIP(hipSetDevice(c->device));
   return pm_adjoint_dev(c, nsteps, dt, cfg, nsens, sensor_dof, target, ckpt_every, J, grad_logE,
                         grad_logk);

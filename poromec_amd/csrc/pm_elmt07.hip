@@ -6,7 +6,8 @@
 // IMPLEMENTADO") and sizes a maxno-8 hexa8 mesh for it (nst = 32,
 // Rdata.f:650-660); it is elmt17_pm's weak form (Elmlib03.f:2022-2256) with
 // trilinear displacements: Kuu = int B^T D B, Kpu = -alpha int N div N_u,
-// Kpp = -(int N N / M + dt perm int grad N . grad N).
+// Kpp = -(int N N / M + dt perm int grad N . grad N), and optionally a
+// pressure stabilization from the material's e(9) (k_q1_isw2).
 //
 // An element has 8 vertices, so one wavefront takes one element and a lane
 // one vertex pair (a, b): its 4 x 4 block (3 x 3 Kuu, 3 Kup, 3 Kpu, Kpp)
@@ -81,6 +82,17 @@ __global__ __launch_bounds__(256) void k_q1_isw2(ElemArgs A) {
   const int a = lane >> 3, b = lane & 7;
   if (live) {
     const ElemMat M = elem_material(S.mat, A.dt, A.gmod);
+    // pressure stabilization, e(9) = beta (0: none): -tau int grad N . grad N
+    // in Kpp, tau = beta h^2 / (lambda + 2 mu), h = V^(1/3) -- the perturbation
+    // tau d/dt (div grad p) of the flow equation (Aguilar, Gaspar, Lisbona,
+    // Rodrigo, IJNME 75 (2008) 1282-1300), on the step's pressure change like
+    // the storage term
+    double tau = 0.0;
+    if (S.mat[8] != 0.0) {
+      double vol = 0.0;
+      for (int g = 0; g < 8; g++) vol = vol + S.w[g];
+      tau = S.mat[8] * cbrt(vol) * cbrt(vol) / M.a;
+    }
     double kuu[9], kup[3], kpu[3], kpp = 0.0;
     for (int i = 0; i < 9; i++) kuu[i] = 0.0;
     for (int i = 0; i < 3; i++) kup[i] = kpu[i] = 0.0;
@@ -110,6 +122,7 @@ __global__ __launch_bounds__(256) void k_q1_isw2(ElemArgs A) {
       kup[2] = kup[2] - zi * wqb;
       const double w1 = w * M.imod_biot, w2 = w * M.dt_perm;
       kpp = kpp - (w1 * ha * hb + w2 * (xi * xj + yi * yj + zi * zj));
+      if (tau != 0.0) kpp = kpp - (w * tau) * (xi * xj + yi * yj + zi * zj);
     }
     for (int i = 0; i < 3; i++) {
       for (int j = 0; j < 3; j++) s_K[wv][3 * a + i][3 * b + j] = kuu[3 * i + j];

@@ -344,6 +344,161 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_cha(
   }
 }
 
+// Runs of <= 4 consecutive columns, one per lane, 32 lanes per block: a
+// wavefront takes two consecutive blocks (k, k + 1), one per half.  Blocks of
+// <= 32 runs -- the Q1/Q1 element's vertices have 27 -- take one pass and
+// keep 27 of 32 lanes busy instead of 27 of 64; wider blocks take more
+// passes.  The two blocks' values and descriptors lie one after the other,
+// so one resource based at the first block's covers both and the second
+// half adds its block's offset; rows past a block's count read the next
+// block's values (or zero past the second) and are never written.  The
+// next pair's descriptors are loaded during the first pass.  XCD sweep
+// mapping over pairs.
+struct PmPairMeta {
+  PmRunMeta a, b;
+};
+
+__device__ __forceinline__ PmPairMeta pm_pair_meta(int64_t p, int64_t p1, int nrn,
+                                                   const int64_t *__restrict__ colptr,
+                                                   const int64_t *__restrict__ valptr,
+                                                   const int *__restrict__ nr,
+                                                   const int64_t *__restrict__ chptr) {
+  PmPairMeta m;
+  const int64_t k = p < p1 ? 2 * p : (int64_t)nrn;  // past the range: two empty blocks
+  m.a = pm_run_meta(k, nrn, colptr, valptr, nr, chptr);
+  m.b = pm_run_meta(k + 1, nrn, colptr, valptr, nr, chptr);
+  return m;
+}
+
+template <typename VT, int NV, bool SCALE_M>
+__global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_ch2(
+    int nrn, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
+    const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
+    const int64_t *__restrict__ chptr, const unsigned int *__restrict__ chunk,
+    const VT *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
+    const double *__restrict__ w, const double *__restrict__ m, double *partials,
+    const PmScalars *S) {
+  if (S != nullptr && S->done) return;
+  constexpr int GROUPS = PM_WG / 64;
+  constexpr int NT = 2;
+  constexpr int VPR = sizeof(VT) == 4 ? 1 : 2;  // 16-byte loads per row and run
+  const __amdgpu_buffer_rsrc_t xr = pm_rsrc(x, (unsigned int)neq * 8u);
+  const int lane = threadIdx.x % 64;
+  const int half = lane >> 5, hl = lane & 31;
+  const unsigned long long hmask = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+  const int grp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+  const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
+  const int64_t npair = ((int64_t)nrn + 1) / 2;
+  int64_t p0, p1, pfirst, pstep;
+  if ((gridDim.x & 7) == 0) {
+    const int W = (int)(gridDim.x >> 3);
+    pm_chunk(npair, lb / W, 8, p0, p1);
+    pfirst = p0 + (int64_t)(lb % W) * GROUPS + grp;
+    pstep = (int64_t)W * GROUPS;
+  } else {
+    pm_chunk(npair, lb, gridDim.x, p0, p1);
+    pfirst = p0 + grp;
+    pstep = GROUPS;
+  }
+  double d0 = 0.0, d1 = 0.0;
+  PmPairMeta cur = pm_pair_meta(pfirst, p1, nrn, colptr, valptr, nr, chptr);
+  PmPairMeta nx = pm_pair_meta(pfirst + pstep, p1, nrn, colptr, valptr, nr, chptr);
+  unsigned int dsc;
+  {
+    const __amdgpu_buffer_rsrc_t h0 =
+        pm_rsrc(chunk + cur.a.hp, (unsigned int)(cur.a.nh + cur.b.nh) * 4u);
+    const unsigned int ho = half ? (unsigned int)(cur.b.hp - cur.a.hp) : 0u;
+    dsc = __builtin_amdgcn_raw_buffer_load_b32(h0, (ho + (unsigned int)hl) * 4u, 0, NT);
+  }
+  for (int64_t p = pfirst; p < p1; p += pstep) {
+    const PmPairMeta nn = pm_pair_meta(p + 2 * pstep, p1, nrn, colptr, valptr, nr, chptr);
+    const int64_t k = 2 * p + half;
+    const PmRunMeta &me = half ? cur.b : cur.a;
+    const int n = me.n, nh = me.nh;
+    const int f = n > 0 ? fr[k] : 0;
+    double wq[4] = {0.0, 0.0, 0.0, 0.0}, mq[4] = {1.0, 1.0, 1.0, 1.0};
+    if (hl == 31) {
+      for (int q = 0; q < n; q++) {
+        if (NV >= 1) wq[q] = w[f + q];
+        if (SCALE_M) mq[q] = m[f + q];
+      }
+    }
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    const unsigned int vbytes =
+        ((unsigned int)(cur.a.n * cur.a.c + cur.b.n * cur.b.c) + 3u) * (unsigned int)sizeof(VT);
+    const __amdgpu_buffer_rsrc_t vr = pm_rsrc(val + cur.a.vp, vbytes);
+    const __amdgpu_buffer_rsrc_t hr =
+        pm_rsrc(chunk + cur.a.hp, (unsigned int)(cur.a.nh + cur.b.nh) * 4u);
+    const __amdgpu_buffer_rsrc_t hrn =
+        pm_rsrc(chunk + nx.a.hp, (unsigned int)(nx.a.nh + nx.b.nh) * 4u);
+    const unsigned int voff = half ? (unsigned int)(cur.b.vp - cur.a.vp) : 0u;
+    const unsigned int hoff = half ? (unsigned int)(cur.b.hp - cur.a.hp) : 0u;
+    const unsigned int hoffn = half ? (unsigned int)(nx.b.hp - nx.a.hp) : 0u;
+    const unsigned int rowb = (unsigned int)me.c * (unsigned int)sizeof(VT);
+    const int passes = max(cur.a.nh, cur.b.nh);  // wave-uniform trip count
+    unsigned int base = 0, dnext = 0;
+    for (int h0 = 0; h0 < passes; h0 += 32) {
+      const unsigned int d =
+          h0 == 0 ? dsc
+                  : __builtin_amdgcn_raw_buffer_load_b32(hr, (hoff + (unsigned int)(h0 + hl)) * 4u,
+                                                         0, NT);
+      const bool valid = h0 + hl < nh;
+      const unsigned int l1 = d & 3u;
+      const unsigned long long bv = __ballot(valid) & hmask;
+      const unsigned long long b0 = __ballot(valid && (l1 & 1u)) & hmask;
+      const unsigned long long b1 = __ballot(valid && (l1 & 2u)) & hmask;
+      const unsigned int off =
+          base + pm_lanes_below(bv) + pm_lanes_below(b0) + 2u * pm_lanes_below(b1);
+      const int len = valid ? (int)l1 + 1 : 0;
+      base += (unsigned int)(__popcll(bv) + __popcll(b0) + 2 * __popcll(b1));
+      typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+      u4 wv[4][VPR];
+      const unsigned int o = (voff + off) * (unsigned int)sizeof(VT);
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int hh = 0; hh < VPR; hh++)
+          wv[q][hh] = __builtin_bit_cast(
+              u4, __builtin_amdgcn_raw_buffer_load_b128(vr, o + (unsigned int)q * rowb + 16u * hh,
+                                                        0, NT));
+      double xv[4];
+      const unsigned int col = d >> 2;
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) {
+        const double xc = pm_bload<double, 0>(xr, (col + (unsigned int)kk) * 8u);
+        xv[kk] = kk < len ? xc : 0.0;
+      }
+      if (h0 == 0)  // the next pair's first descriptors, behind this pass's loads
+        dnext = __builtin_amdgcn_raw_buffer_load_b32(hrn, (hoffn + (unsigned int)hl) * 4u, 0, NT);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        VT v[4];
+        __builtin_memcpy(v, &wv[q][0], sizeof(v));
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) a[q] += (double)v[kk] * xv[kk];
+      }
+    }
+    dsc = dnext;
+#pragma unroll
+    for (int q = 0; q < 4; q++) a[q] = pm_group_sum<32>(a[q]);
+    if (hl == 31) {
+      for (int q = 0; q < n; q++) {
+        double yq = a[q];
+        if (SCALE_M) yq *= mq[q];
+        y[f + q] = yq;
+        if (NV >= 1) d0 += yq * wq[q];
+        if (NV >= 2) d1 += yq * yq;
+      }
+    }
+    cur = nx;
+    nx = nn;
+  }
+  if (NV > 0) {
+    double v[2] = {d0, d1};
+    pm_block_partials<NV>(v, partials);
+  }
+}
+
 // Sums the workgroup partials of the kernel before it (fixed order) into
 // S->red and runs the epilogue that computes the next solver scalars.
 template <int NV, class Epi>
@@ -415,6 +570,11 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
     PM_SPMV_CH_CASE(10, 2, false)
     PM_SPMV_CH_CASE(11, 1, true)
     PM_SPMV_CH_CASE(12, 2, true)
+    case 15:
+      hipLaunchKernelGGL((k_spmv_ch2<VT, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream, A.nrn,
+                         A.neq, A.fr, A.nr, A.colptr, A.valptr, A.chptr, A.chunk, val, x, y, w, m,
+                         c->d_partials, S);
+      break;
     default: PM_SPMV_CASE(5, 32, 4)
   }
 #undef PM_SPMV_CASE

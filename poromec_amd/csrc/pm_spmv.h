@@ -17,11 +17,13 @@
 // PF 2: one run of <= 4 consecutive columns per lane, UNR runs in flight
 // (k_spmv_ch); PF 3: the same with the XCD's workgroups sweeping its blocks
 // together; PF 4: PF 1 sweeping; PF 5: runs, one or two per lane by block
-// width, sweeping (k_spmv_cha)
-#define PM_SPMV_NCFG 15
-static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 64, 64, 32, 64, 64, 64, 64, 64, 64, 64, 64, 64};
-static constexpr int pm_spmv_unr[PM_SPMV_NCFG] = {1, 2, 3, 4, 5, 4, 3, 4, 5, 1, 2, 1, 2, 5, 1};
-static constexpr int pm_spmv_pf[PM_SPMV_NCFG] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 3, 3, 4, 5};
+// width, sweeping (k_spmv_cha); PF 6: runs, one per lane, 32 lanes per block
+// (two blocks per wavefront), sweeping (k_spmv_ch2)
+#define PM_SPMV_NCFG 16
+static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 64, 64, 32, 64, 64,
+                                                    64, 64, 64, 64, 64, 64, 64, 32};
+static constexpr int pm_spmv_unr[PM_SPMV_NCFG] = {1, 2, 3, 4, 5, 4, 3, 4, 5, 1, 2, 1, 2, 5, 1, 1};
+static constexpr int pm_spmv_pf[PM_SPMV_NCFG] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 3, 3, 4, 5, 6};
 
 struct PmBlockMeta {
   int64_t cp, vp;

@@ -22,7 +22,7 @@ from .problem import NDF, NEN, Deck
 
 SOLVERS = {"pcg": 1, "gmres": 2, "bicgstab": 4, "minres": 7, "sqmr": 11}
 SOLVER_NAMES = {v: k for k, v in SOLVERS.items()}
-SPMV_SHAPES = 15  # PM_SPMV_SHAPES: (lanes per node block, columns in flight per lane) choices
+SPMV_SHAPES = 16  # PM_SPMV_SHAPES: (lanes per node block, columns in flight per lane) choices
 
 
 @dataclass

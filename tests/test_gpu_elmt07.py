@@ -144,6 +144,67 @@ def test_q1_march_tform_porosity():
     s.close()
 
 
+def test_q1_stabilized_column():
+    """The benchmark column (4 x 4 x 40, 4100 dofs, M / Kv = 74) with the
+    pressure stabilization beta = 1/4: 20 steps against the oracle's march,
+    and the undrained p0 to 1e-3 below 2 m after the first step.  SQMR runs
+    with the |diag| Jacobi preconditioner (precond 5): with the signed
+    diagonal (precond 2) its recurrence diverges on this system's load
+    vector (the reference then switches to GMRES after maxit, Solv.f:765;
+    random right-hand sides converge), while |diag| converges in ~95
+    iterations."""
+    from poromec_amd.solver import MefSolver, SolverConfig
+    d = P.column_deck(4, 4, 40, dt=0.01, nsteps=20, nen=8)
+    d.e[:, 8] = 0.25
+    m = O.OracleModel(d, precond=5)
+    s = MefSolver(d)
+    E_, nu, M, q = 10.0, 0.3, 1000.0, 1.0e-3
+    Kv = E_ * (1 - nu) / ((1 + nu) * (1 - 2 * nu))
+    p0 = M * q / (Kv + M)
+    z = d.x[:, 2]
+    for k in range(20):
+        io = m.step()
+        ig = s.step(SolverConfig(tol=1e-10, precond=5))
+        assert io["nlit"] == ig["nlit"], (k, io, ig)
+        assert io["used"] == ig["used"][: len(io["used"])] == [11] * len(io["used"])
+        for a, b in zip(io["iters"], ig["iters"]):
+            assert E.close_count(b, a), (k, io["iters"], ig["iters"])
+        u = s.state()["u"]
+        assert _rel(u[:, :3], m.u[:, :3]) <= RTOL_HIST, k
+        assert _rel(u[:, 3], m.u[:, 3]) <= RTOL_HIST, k
+        if k == 0:
+            assert np.abs(u[z <= z.max() - 2.0, 3] / p0 - 1.0).max() < 1e-3
+    s.close()
+
+
+def test_q1_every_spmv_shape():
+    """Every SpMV shape on a Q1/Q1 box (interior node blocks of 4 rows x 108
+    columns, 27 runs): FP64 against the exported CSRC matrix applied on the
+    host, FP32+IR to the FP64 solution."""
+    import scipy.sparse as sp
+    from poromec_amd.solver import SPMV_SHAPES, MefSolver, SolverConfig
+    d = q1_deck(6, 3, loads=False)
+    s = MefSolver(d)
+    s.pform(2, True, False, d.dt, d.dt)
+    ia, ja, ad, al = s.export_csrc()
+    n = s.neq
+    rows = np.repeat(np.arange(n), np.diff(ia))
+    Lm = sp.csr_matrix((al, (rows, ja - 1)), shape=(n, n))
+    x = np.random.default_rng(8).standard_normal(n)
+    y0 = ad * x + Lm @ x + Lm.T @ x
+    b = np.random.default_rng(6).standard_normal(n)
+    x64, _ = s.solve(b, SolverConfig(tol=1e-10))
+    for shape in range(SPMV_SHAPES):
+        s.set_spmv_shape(False, shape)
+        y = s.spmv(x)
+        assert np.abs(y - y0).max() <= 1e-13 * np.abs(y0).max(), shape
+        s.set_spmv_shape(True, shape)
+        x32, i32 = s.solve(b, SolverConfig(tol=1e-10, precision="fp32ir"))
+        assert i32["true_rel"] < 1e-10, shape
+        assert np.abs(x32 - x64).max() <= 1e-8 * np.abs(x64).max(), shape
+    s.close()
+
+
 def test_q1_fp32ir_step():
     from poromec_amd.solver import MefSolver, SolverConfig
     d = q1_deck(6, 2)
@@ -236,4 +297,11 @@ def test_q1_rejects_what_it_does_not_have():
     s.set_ordered(True)
     with pytest.raises(PmError):
         s.pform(2, True, True, d.dt, d.dt)
+    s.close()
+    d.e[:, 8] = 0.25  # the adjoint has no stabilized form
+    s = MefSolver(d)
+    _, idm, _ = s.numbering()
+    free = int(np.nonzero(idm[:, 3] > 0)[0][0])
+    with pytest.raises(PmError):
+        s.adjoint(1, [(free, 3)], np.zeros((1, 1)))
     s.close()
