@@ -35,6 +35,9 @@ module pmdeck
 
   type deck_t
     integer(c_int) :: numel = 0, nnodev = 0, numat = 0
+    ! maxno: 20 (hexa8 promoted to the hexa20/8 element) or 8 (the mesh as
+    ! read, the equal-order Q1/Q1 element, type 7)
+    integer(c_int) :: nen = 20
     integer(c_int), allocatable :: ix(:,:), idv(:,:), ie(:), eload(:,:), eloadp(:,:)
     real(c_double), allocatable :: x(:,:), e(:,:), u0(:,:)
     integer(c_int) :: load(3, 200)
@@ -373,6 +376,7 @@ contains
     if (.not. have_mesh) call fail('deck has no mesh macro')
     if (d%stop_crit /= 1 .and. d%stop_crit /= 2) &
       call fail('only nonlinear stop criteria 1 and 2 are in this path')
+    if (d%nen == 8 .and. d%vprop(1) /= 0) call fail('maxno 8: the Q1/Q1 element has no conseq vprop')
     ! check_element (Rdata.f:3744-3758)
     if ((d%vprop(1) /= 0) .neqv. any(d%ie == 18)) &
       call fail('Erro: Invalid elements!! (elmt18 goes with conseq vprop, elmt17 without)')
@@ -630,7 +634,9 @@ contains
     end do
     if (min(nnode, numel, numat, maxno, ndf, ndim) < 0) call fail('Missing macro in parameters')
     if (ndf /= 4 .or. ndim /= 3) call fail('this path is the 3-D u-p system (ndf 4, dim 3)')
-    if (maxno /= 20) call fail('this path is the hexa20/8 element (maxno 20)')
+    if (maxno /= 20 .and. maxno /= 8) &
+      call fail('this path is the hexa20/8 element (maxno 20) or the Q1/Q1 element (maxno 8)')
+    d%nen = maxno
     d%numel = numel
     d%nnodev = nnode
     d%numat = numat
@@ -722,7 +728,11 @@ contains
     if (.not. (have_el .and. have_x)) call fail('mesh block without hexa8 elements or coordinates')
     if (minval(d%ix(21, :)) < 1 .or. maxval(d%ix(21, :)) > numat) &
       call fail('element material id out of range')
-    where (d%ie == 7) d%ie = 17
+    if (d%nen == 20) then
+      where (d%ie == 7) d%ie = 17
+    else if (any(d%ie /= 7)) then
+      call fail('maxno 8: every material must be element type 7 (Q1/Q1)')
+    end if
     ! boundc (Rdata.f:2300): restrained dofs start at their prescribed values
     where (d%idv > 0) d%u0 = 0.0d0
   end subroutine

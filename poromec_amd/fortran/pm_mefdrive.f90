@@ -158,7 +158,7 @@ contains
   ! precond, maxit, ngram, maxnlit, stop_crit, maxnlit_accept, npnodes,
   ! nprog (int32); tol, nltol, gravity(3), gmod (real64); pnodes, op (int32);
   ! opval (real64); ix, idv, ie, eload, eloadp, load (int32); x, e, u0,
-  ! fload (real64); print_flag(13), legacy_vtk, bvtk (int32)
+  ! fload (real64); print_flag(13), legacy_vtk, bvtk, nen (int32)
   subroutine dump_deck(path, prefix)
     character(len=*), intent(in) :: path, prefix
     type(deck_t) :: d
@@ -175,7 +175,7 @@ contains
     write(u) d%ix, d%idv, d%ie, d%eload, d%eloadp, d%load
     write(u) d%x, d%e, d%u0, d%fload
     write(u) merge(1_c_int, 0_c_int, d%fprint), merge(1_c_int, 0_c_int, d%vtk_legacy), &
-      merge(1_c_int, 0_c_int, d%vtk_binary)
+      merge(1_c_int, 0_c_int, d%vtk_binary), d%nen
     close(u)
     write(*, '(a,i10,a,i10,a,i8)') ' parsed: numel', d%numel, ' nnodev', d%nnodev, ' ops', d%nprog
   end subroutine
@@ -232,7 +232,8 @@ contains
     integer(c_int) :: nnode, neq, nb, reorder, ierr
     integer(c_int64_t) :: nad, nnz, ncols
     real(c_double), allocatable :: u(:), u0(:), dp(:), poro(:), tx(:), txe(:), txb(:), flux(:)
-    integer(c_int), allocatable :: ix20(:), idn(:), perm(:)
+    integer(c_int), allocatable :: ix20(:), idn(:), perm(:), ix9(:,:)
+    logical :: fres(13)
     real(c_double) :: dt, t, tnow, ms_elem, ms_solve, ms_total
     integer :: iop, istep, k, flog, fnl, ftm, nsolv
     integer(8) :: c0, c1, crate
@@ -246,8 +247,17 @@ contains
     reorder = merge(1_c_int, 0_c_int, d%reorder)
     ierr = pm_ctx_create(ctx, 0_c_int)
     call check(ierr, 'pm_ctx_create')
-    ierr = pm_mesh_setup(ctx, d%numel, d%nnodev, d%ix, d%x, d%idv, reorder, nnode, neq, nad)
-    call check(ierr, 'pm_mesh_setup')
+    if (d%nen == 8) then  ! maxno 8: ix (9, numel), the Q1/Q1 element on the mesh as read
+      allocate(ix9(9, d%numel))
+      ix9(1:8, :) = d%ix(1:8, :)
+      ix9(9, :) = d%ix(21, :)
+      ierr = pm_mesh_setup_nen(ctx, 8_c_int, d%numel, d%nnodev, ix9, d%x, d%idv, reorder, &
+                               nnode, neq, nad)
+      call check(ierr, 'pm_mesh_setup_nen')
+    else
+      ierr = pm_mesh_setup(ctx, d%numel, d%nnodev, d%ix, d%x, d%idv, reorder, nnode, neq, nad)
+      call check(ierr, 'pm_mesh_setup')
+    end if
     ierr = pm_set_materials(ctx, d%numat, d%e, d%eload, d%eloadp, d%load, d%fload, d%gravity, &
                             d%gmod)
     call check(ierr, 'pm_set_materials')
@@ -368,8 +378,10 @@ contains
           ierr = pm_get_numbering(ctx, ix20, idn, perm)
           call check(ierr, 'pm_get_numbering')
         end if
+        fres = d%fprint
+        if (d%nen == 8) fres(1) = .false.  ! maxno 8: vertices only, no quadratic cells
         call write_res(res_name(prefix, istep, d%vtk_legacy), ix20, d%x, d%nnodev, u, dp, poro, &
-                       tx, txb, txe, flux, nnode, d%numel, istep, t, d%fprint, d%vtk_legacy, &
+                       tx, txb, txe, flux, nnode, d%numel, istep, t, fres, d%vtk_legacy, &
                        d%vtk_binary, k)
         if (k /= 0) then
           write(*, '(a,i3)') ' *** pres: results file not written, code', k

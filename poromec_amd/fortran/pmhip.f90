@@ -62,6 +62,16 @@ module pmhip
       integer(c_int) :: nnode, neq
       integer(c_int64_t) :: nad
     end function
+    integer(c_int) function pm_mesh_setup_nen(ctx, nen, numel, nnodev, ix, x, idv, reorder, &
+                                              nnode, neq, nad) bind(C, name='pm_mesh_setup_nen')
+      import :: c_ptr, c_int, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nen, numel, nnodev, reorder
+      integer(c_int) :: ix(*), idv(*)
+      real(c_double) :: x(*)
+      integer(c_int) :: nnode, neq
+      integer(c_int64_t) :: nad
+    end function
     integer(c_int) function pm_mesh_numbering(numel, nnodev, ix, idv, reorder, nnode_max, ix20, &
                                               id, perm, nnode, neq, nad) &
         bind(C, name='pm_mesh_numbering')

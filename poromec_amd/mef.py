@@ -138,7 +138,9 @@ def run(inp: MefInput, prefix: str, precision: str = "fp64", ordered: bool = Fal
                         ix20 = s.numbering()[0]
                     vtk.write_results(vtk.results_name(prefix, istep, inp.vtk_legacy), ix20,
                                       deck.x, st["u"], st["dp"], st["porosity"], tx, txb, txe,
-                                      flux, istep, t, flags=fl, quad=fl[0], legacy=inp.vtk_legacy,
+                                      flux, istep, t, flags=fl,
+                                      quad=fl[0] and deck.nen == 20,  # maxno 8: vertices only
+                                      legacy=inp.vtk_legacy,
                                       binary=inp.vtk_binary, nnodev=deck.nnodev)
                 elif op[0] == "solv":
                     istep += 1

@@ -61,9 +61,10 @@ def fparse(path, tmp):
     out["e"] = take(np.float64, 15 * numat).reshape(numat, 15)
     out["u0"] = take(np.float64, 4 * nnodev).reshape(nnodev, 4)
     out["fload"] = take(np.float64, 1000 * 5 * 200).reshape(200, 5, 1000)
-    fl = take(np.int32, 15)
+    fl = take(np.int32, 16)
     out["print_flags"] = tuple(bool(v) for v in fl[:13])
     out["vtk_legacy"], out["vtk_binary"] = bool(fl[13]), bool(fl[14])
+    out["nen"] = int(fl[15])
     assert pos == len(b)
     return out, r.stdout
 
@@ -71,6 +72,7 @@ def fparse(path, tmp):
 def assert_same(f, inp):
     d, c = inp.deck, inp.cfg
     assert (f["numel"], f["nnodev"], f["numat"]) == (d.numel, d.nnodev, d.e.shape[0])
+    assert f["nen"] == d.nen
     assert f["reorder"] == inp.reorder
     assert f["vprop"] == tuple(bool(v) for v in inp.vprop)
     assert f["solver"] == SOLVERS[c.solver]
@@ -104,8 +106,8 @@ def test_example_decks(tmp_path, case):
     assert_same(f, read_deck(path))
 
 
-def _written(tmp_path, **kw):
-    deck = P.column_deck(2, 2, 3, dt=0.05)
+def _written(tmp_path, nen=20, **kw):
+    deck = P.column_deck(2, 2, 3, dt=0.05, nen=nen)
     deck.load[4] = [39, 2, 3]  # a time-table load from a file
     deck.fload[4, 0, :3] = [0.0, 1.0, 2.0]
     deck.fload[4, 1, :3] = [-1.5e3, 2.25e3, 3.0e3]
@@ -125,6 +127,23 @@ def test_written_deck(tmp_path):
     inp = read_deck(path)
     assert inp.reorder is False and inp.cfg.solver == "minres"
     assert_same(f, inp)
+
+
+def test_written_deck_q1(tmp_path):
+    """A maxno-8 deck (the Q1/Q1 element, type 7) reads the same in both
+    readers; a maxno-8 deck with an element-17 material stops the Fortran
+    reader."""
+    path = _written(tmp_path, nen=8)
+    f, log = fparse(path, tmp_path)
+    assert f is not None, log
+    inp = read_deck(path)
+    assert inp.deck.nen == 8 and f["nen"] == 8 and f["ie"].tolist() == [7]
+    assert_same(f, inp)
+    path = Path(path)
+    text = path.read_text().replace("\n1 7 ", "\n1 17 ")
+    path.write_text(text)
+    f, log = fparse(path, tmp_path)
+    assert f is None and "maxno 8" in log
 
 
 def test_written_deck_vprop(tmp_path):

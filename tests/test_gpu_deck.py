@@ -2,6 +2,8 @@
 reader and the output writers (poromec_amd.mef): the up_node files must hold
 the oracle's histories of the same deck (1e-9, and bit for bit in reference
 summation order) and the reference's own printed results."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -107,3 +109,33 @@ def test_deck_fp32ir(tmp_path):
     for n in inp.pnodes:
         g, r = res["hist"][n], ref["hist"][n][: nsteps + 1]
         assert (np.abs(g[:, 2:] - r[:, 2:]) <= 1e-8 * scale + 1e-9).all(), n
+
+
+def test_q1_deck_run(tmp_path):
+    """A maxno-8 deck (the Q1/Q1 element, type 7) written by write_deck and
+    run end to end by the deck runner: the up_node histories hold the
+    oracle's march of the same deck to 1e-9, and a results file is written
+    with linear hexa8 cells."""
+    from poromec_amd import problem as P
+    from poromec_amd.deck import write_deck
+    from poromec_amd.solver import SolverConfig
+    d = P.heterogeneous_box_deck(4, seed=7, dt=0.1, nsteps=4, nen=8)
+    path = write_deck(tmp_path / "q1.dat", d, SolverConfig(stop_crit=1, tol=1e-11),
+                      pnodes=(1, 30, 60), nsteps=4)
+    inp = read_deck(path)
+    assert inp.deck.nen == 8
+    inp = dataclasses.replace(inp, program=inp.program + [("pres",)])
+    prefix = str(tmp_path / "q1")
+    res = run(inp, prefix)
+    ohist, oiters, _ = oracle_run(inp, 4)
+    allo = np.concatenate([ohist[n][:, 2:] for n in inp.pnodes])
+    scale = np.array([np.abs(allo[:, :3]).max()] * 3 + [np.abs(allo[:, 3]).max()])
+    for n in inp.pnodes:
+        assert (np.abs(res["hist"][n][:, 2:] - ohist[n][:, 2:]) <= 1e-9 * scale).all(), n
+    import glob
+    vtu = sorted(glob.glob(prefix + "_istep_*.vtu"))
+    assert vtu
+    text = open(vtu[-1]).read()
+    types = text.split('Name="types" format="ascii">')[1].split("</DataArray>")[0].split()
+    assert types == ["12"] * d.numel  # VTK_HEXAHEDRON: 8 vertices per cell
+    assert f'NumberOfPoints="{d.nnodev:15d}"' in text

@@ -72,6 +72,35 @@ def test_fortran_host_fp32ir(tmp_path):
     _run_both(path, tmp_path, 0, fp32ir=True)
 
 
+def test_fortran_host_q1_deck(tmp_path):
+    """A maxno-8 deck (the Q1/Q1 element, type 7) through the Fortran host:
+    histories, stresses and logs as the Python runner writes them, and a
+    results file with the setprint word quadratic still in linear hexa8
+    cells (the mesh has no mid-edge nodes)."""
+    from poromec_amd import vtk as V
+    print("written Q1/Q1 deck, 4 steps", flush=True)
+    deck = P.heterogeneous_box_deck(3, seed=4, dt=0.05, nsteps=4, nen=8)
+    path = write_deck(tmp_path / "q1.dat", deck, SolverConfig(stop_crit=1), pnodes=(1, 20),
+                      nsteps=4)
+    (tmp_path / "q1_print.dat").write_text("quadratic\ndesloc\npressure\ntotalstress\nend\n")
+    text = Path(path).read_text().replace("mesh\n", "setprint q1_print.dat\nmesh\n", 1)
+    Path(path).write_text(text.replace("pnup\nnext", "pnup\npnsf\npres\nnext"))
+    inp, fo = _run_both(path, tmp_path, 0)
+    assert inp.deck.nen == 8 and inp.print_flags[0]
+    py = str(tmp_path / "py")
+    a = Path(V.results_name(py, 4, False)).read_bytes()
+    b = Path(V.results_name(fo, 4, False)).read_bytes()
+    la, lb = a.split(b"\n"), b.split(b"\n")
+    for j in range(len(la)):
+        if b'Name="TIME_' in la[j]:
+            assert float(la[j + 1]) == float(lb[j + 1])
+            la[j + 1] = lb[j + 1]
+    assert b"\n".join(la) == b
+    t = b.decode()
+    types = t.split('Name="types" format="ascii">')[1].split("</DataArray>")[0].split()
+    assert types == ["12"] * deck.numel
+
+
 def test_fortran_host_vprop_deck(tmp_path):
     """elmt18 (properties at the integration points, Kozeny-Carman and
     Hashin-Shtrikman) and a time-table load through the Fortran host."""
