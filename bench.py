@@ -52,7 +52,7 @@ def spmv_kernel_name(prec, shape):
     """The SpMV kernel the solver launches at this shape, with its fused dot."""
     vt = "float" if prec == "fp32" else "double"
     if shape_pf[shape] == 6:
-        return f"k_spmv_ch2<{vt}, 1, false>"
+        return f"k_spmv_chg<{vt}, {shape_lanes[shape]}, {shape_unr[shape]}, 1, false>"
     if shape_pf[shape] == 5:
         return f"k_spmv_cha<{vt}, 1, false>"
     if shape_pf[shape] == 4:

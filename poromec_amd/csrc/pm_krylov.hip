@@ -344,34 +344,49 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_cha(
   }
 }
 
-// Runs of <= 4 consecutive columns, one per lane, 32 lanes per block: a
-// wavefront takes two consecutive blocks (k, k + 1), one per half.  Blocks of
-// <= 32 runs -- the Q1/Q1 element's vertices have 27 -- take one pass and
-// keep 27 of 32 lanes busy instead of 27 of 64; wider blocks take more
-// passes.  The two blocks' values and descriptors lie one after the other,
-// so one resource based at the first block's covers both and the second
-// half adds its block's offset; rows past a block's count read the next
-// block's values (or zero past the second) and are never written.  The
-// next pair's descriptors are loaded during the first pass.  XCD sweep
-// mapping over pairs.
-struct PmPairMeta {
-  PmRunMeta a, b;
+// Runs of <= 4 consecutive columns with LPB lanes per block: a wavefront
+// takes G = 64 / LPB consecutive blocks, one per group of LPB lanes, and
+// each lane U runs per pass.  Shape 15 is (LPB, U) = (32, 1); (16, 2), four
+// blocks per wavefront, measured 1.8x slower on the Q1/Q1 128^3 box and is
+// not offered.  Blocks of <= LPB * U runs -- the
+// Q1/Q1 element's vertices have 27 -- take one pass and keep 27 of 32 run
+// slots busy instead of 27 of 64; wider blocks take more passes.  The G
+// blocks' values and descriptors lie one after the other, so one resource
+// based at the first block's covers them all and each group adds its
+// block's offset; rows past a block's count read the next block's values
+// (or zero past the last) and are never written.  The next group of
+// blocks' descriptors are loaded during the first pass.  XCD sweep mapping
+// over the groups of blocks.
+template <int G>
+struct PmBlocksMeta {
+  PmRunMeta b[G];
 };
 
-__device__ __forceinline__ PmPairMeta pm_pair_meta(int64_t p, int64_t p1, int nrn,
-                                                   const int64_t *__restrict__ colptr,
-                                                   const int64_t *__restrict__ valptr,
-                                                   const int *__restrict__ nr,
-                                                   const int64_t *__restrict__ chptr) {
-  PmPairMeta m;
-  const int64_t k = p < p1 ? 2 * p : (int64_t)nrn;  // past the range: two empty blocks
-  m.a = pm_run_meta(k, nrn, colptr, valptr, nr, chptr);
-  m.b = pm_run_meta(k + 1, nrn, colptr, valptr, nr, chptr);
+template <int G>
+__device__ __forceinline__ PmBlocksMeta<G> pm_blocks_meta(int64_t p, int64_t p1, int nrn,
+                                                          const int64_t *__restrict__ colptr,
+                                                          const int64_t *__restrict__ valptr,
+                                                          const int *__restrict__ nr,
+                                                          const int64_t *__restrict__ chptr) {
+  PmBlocksMeta<G> m;
+  const int64_t k = p < p1 ? (int64_t)G * p : (int64_t)nrn;  // past the range: empty blocks
+#pragma unroll
+  for (int j = 0; j < G; j++) m.b[j] = pm_run_meta(k + j, nrn, colptr, valptr, nr, chptr);
   return m;
 }
 
-template <typename VT, int NV, bool SCALE_M>
-__global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_ch2(
+// the block of lane group gi among the G scalar metadata sets
+template <int G>
+__device__ __forceinline__ PmRunMeta pm_pick(const PmBlocksMeta<G> &m, int gi) {
+  PmRunMeta r = m.b[0];
+#pragma unroll
+  for (int j = 1; j < G; j++)
+    if (gi == j) r = m.b[j];
+  return r;
+}
+
+template <typename VT, int LPB, int U, int NV, bool SCALE_M>
+__global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_chg(
     int nrn, int neq, const int *__restrict__ fr, const int *__restrict__ nr,
     const int64_t *__restrict__ colptr, const int64_t *__restrict__ valptr,
     const int64_t *__restrict__ chptr, const unsigned int *__restrict__ chunk,
@@ -379,109 +394,139 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_ch2(
     const double *__restrict__ w, const double *__restrict__ m, double *partials,
     const PmScalars *S) {
   if (S != nullptr && S->done) return;
+  constexpr int G = 64 / LPB;  // blocks per wavefront
   constexpr int GROUPS = PM_WG / 64;
   constexpr int NT = 2;
   constexpr int VPR = sizeof(VT) == 4 ? 1 : 2;  // 16-byte loads per row and run
   const __amdgpu_buffer_rsrc_t xr = pm_rsrc(x, (unsigned int)neq * 8u);
   const int lane = threadIdx.x % 64;
-  const int half = lane >> 5, hl = lane & 31;
-  const unsigned long long hmask = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+  const int gi = lane / LPB, gl = lane % LPB;
+  const unsigned long long gmask = (LPB == 64 ? ~0ull : ((1ull << LPB) - 1ull)) << (gi * LPB);
   const int grp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
   const int lb = pm_xcd_block(blockIdx.x, gridDim.x);
-  const int64_t npair = ((int64_t)nrn + 1) / 2;
+  const int64_t nitem = ((int64_t)nrn + G - 1) / G;
   int64_t p0, p1, pfirst, pstep;
   if ((gridDim.x & 7) == 0) {
     const int W = (int)(gridDim.x >> 3);
-    pm_chunk(npair, lb / W, 8, p0, p1);
+    pm_chunk(nitem, lb / W, 8, p0, p1);
     pfirst = p0 + (int64_t)(lb % W) * GROUPS + grp;
     pstep = (int64_t)W * GROUPS;
   } else {
-    pm_chunk(npair, lb, gridDim.x, p0, p1);
+    pm_chunk(nitem, lb, gridDim.x, p0, p1);
     pfirst = p0 + grp;
     pstep = GROUPS;
   }
   double d0 = 0.0, d1 = 0.0;
-  PmPairMeta cur = pm_pair_meta(pfirst, p1, nrn, colptr, valptr, nr, chptr);
-  PmPairMeta nx = pm_pair_meta(pfirst + pstep, p1, nrn, colptr, valptr, nr, chptr);
-  unsigned int dsc;
+  PmBlocksMeta<G> cur = pm_blocks_meta<G>(pfirst, p1, nrn, colptr, valptr, nr, chptr);
+  PmBlocksMeta<G> nx = pm_blocks_meta<G>(pfirst + pstep, p1, nrn, colptr, valptr, nr, chptr);
+  auto desc_bytes = [](const PmBlocksMeta<G> &mm) {
+    int nh = 0;
+#pragma unroll
+    for (int j = 0; j < G; j++) nh += mm.b[j].nh;
+    return (unsigned int)nh * 4u;
+  };
+  unsigned int dsc[U];
   {
-    const __amdgpu_buffer_rsrc_t h0 =
-        pm_rsrc(chunk + cur.a.hp, (unsigned int)(cur.a.nh + cur.b.nh) * 4u);
-    const unsigned int ho = half ? (unsigned int)(cur.b.hp - cur.a.hp) : 0u;
-    dsc = __builtin_amdgcn_raw_buffer_load_b32(h0, (ho + (unsigned int)hl) * 4u, 0, NT);
+    const __amdgpu_buffer_rsrc_t h0 = pm_rsrc(chunk + cur.b[0].hp, desc_bytes(cur));
+    const unsigned int ho = (unsigned int)(pm_pick<G>(cur, gi).hp - cur.b[0].hp);
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      dsc[u] = __builtin_amdgcn_raw_buffer_load_b32(h0, (ho + (unsigned int)(u * LPB + gl)) * 4u,
+                                                    0, NT);
   }
   for (int64_t p = pfirst; p < p1; p += pstep) {
-    const PmPairMeta nn = pm_pair_meta(p + 2 * pstep, p1, nrn, colptr, valptr, nr, chptr);
-    const int64_t k = 2 * p + half;
-    const PmRunMeta &me = half ? cur.b : cur.a;
+    const PmBlocksMeta<G> nn = pm_blocks_meta<G>(p + 2 * pstep, p1, nrn, colptr, valptr, nr, chptr);
+    const int64_t k = (int64_t)G * p + gi;
+    const PmRunMeta me = pm_pick<G>(cur, gi);
     const int n = me.n, nh = me.nh;
     const int f = n > 0 ? fr[k] : 0;
     double wq[4] = {0.0, 0.0, 0.0, 0.0}, mq[4] = {1.0, 1.0, 1.0, 1.0};
-    if (hl == 31) {
+    if (gl == LPB - 1) {
       for (int q = 0; q < n; q++) {
         if (NV >= 1) wq[q] = w[f + q];
         if (SCALE_M) mq[q] = m[f + q];
       }
     }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    const unsigned int vbytes =
-        ((unsigned int)(cur.a.n * cur.a.c + cur.b.n * cur.b.c) + 3u) * (unsigned int)sizeof(VT);
-    const __amdgpu_buffer_rsrc_t vr = pm_rsrc(val + cur.a.vp, vbytes);
-    const __amdgpu_buffer_rsrc_t hr =
-        pm_rsrc(chunk + cur.a.hp, (unsigned int)(cur.a.nh + cur.b.nh) * 4u);
-    const __amdgpu_buffer_rsrc_t hrn =
-        pm_rsrc(chunk + nx.a.hp, (unsigned int)(nx.a.nh + nx.b.nh) * 4u);
-    const unsigned int voff = half ? (unsigned int)(cur.b.vp - cur.a.vp) : 0u;
-    const unsigned int hoff = half ? (unsigned int)(cur.b.hp - cur.a.hp) : 0u;
-    const unsigned int hoffn = half ? (unsigned int)(nx.b.hp - nx.a.hp) : 0u;
-    const unsigned int rowb = (unsigned int)me.c * (unsigned int)sizeof(VT);
-    const int passes = max(cur.a.nh, cur.b.nh);  // wave-uniform trip count
-    unsigned int base = 0, dnext = 0;
-    for (int h0 = 0; h0 < passes; h0 += 32) {
-      const unsigned int d =
-          h0 == 0 ? dsc
-                  : __builtin_amdgcn_raw_buffer_load_b32(hr, (hoff + (unsigned int)(h0 + hl)) * 4u,
-                                                         0, NT);
-      const bool valid = h0 + hl < nh;
-      const unsigned int l1 = d & 3u;
-      const unsigned long long bv = __ballot(valid) & hmask;
-      const unsigned long long b0 = __ballot(valid && (l1 & 1u)) & hmask;
-      const unsigned long long b1 = __ballot(valid && (l1 & 2u)) & hmask;
-      const unsigned int off =
-          base + pm_lanes_below(bv) + pm_lanes_below(b0) + 2u * pm_lanes_below(b1);
-      const int len = valid ? (int)l1 + 1 : 0;
-      base += (unsigned int)(__popcll(bv) + __popcll(b0) + 2 * __popcll(b1));
-      typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-      u4 wv[4][VPR];
-      const unsigned int o = (voff + off) * (unsigned int)sizeof(VT);
+    unsigned int nvals = 3u;
+    int passes = 0;
 #pragma unroll
-      for (int q = 0; q < 4; q++)
-#pragma unroll
-        for (int hh = 0; hh < VPR; hh++)
-          wv[q][hh] = __builtin_bit_cast(
-              u4, __builtin_amdgcn_raw_buffer_load_b128(vr, o + (unsigned int)q * rowb + 16u * hh,
-                                                        0, NT));
-      double xv[4];
-      const unsigned int col = d >> 2;
-#pragma unroll
-      for (int kk = 0; kk < 4; kk++) {
-        const double xc = pm_bload<double, 0>(xr, (col + (unsigned int)kk) * 8u);
-        xv[kk] = kk < len ? xc : 0.0;
-      }
-      if (h0 == 0)  // the next pair's first descriptors, behind this pass's loads
-        dnext = __builtin_amdgcn_raw_buffer_load_b32(hrn, (hoffn + (unsigned int)hl) * 4u, 0, NT);
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        VT v[4];
-        __builtin_memcpy(v, &wv[q][0], sizeof(v));
-#pragma unroll
-        for (int kk = 0; kk < 4; kk++) a[q] += (double)v[kk] * xv[kk];
-      }
+    for (int j = 0; j < G; j++) {
+      nvals += (unsigned int)(cur.b[j].n * cur.b[j].c);
+      passes = max(passes, cur.b[j].nh);  // wave-uniform trip count
     }
-    dsc = dnext;
+    const __amdgpu_buffer_rsrc_t vr = pm_rsrc(val + cur.b[0].vp, nvals * (unsigned int)sizeof(VT));
+    const __amdgpu_buffer_rsrc_t hr = pm_rsrc(chunk + cur.b[0].hp, desc_bytes(cur));
+    const __amdgpu_buffer_rsrc_t hrn = pm_rsrc(chunk + nx.b[0].hp, desc_bytes(nx));
+    const unsigned int voff = (unsigned int)(me.vp - cur.b[0].vp);
+    const unsigned int hoff = (unsigned int)(me.hp - cur.b[0].hp);
+    const unsigned int hoffn = (unsigned int)(pm_pick<G>(nx, gi).hp - nx.b[0].hp);
+    const unsigned int rowb = (unsigned int)me.c * (unsigned int)sizeof(VT);
+    unsigned int base = 0, dnext[U];
 #pragma unroll
-    for (int q = 0; q < 4; q++) a[q] = pm_group_sum<32>(a[q]);
-    if (hl == 31) {
+    for (int u = 0; u < U; u++) dnext[u] = 0;
+    for (int h0 = 0; h0 < passes; h0 += LPB * U) {
+      unsigned int d[U], off[U];
+      int len[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const unsigned int r = (unsigned int)(h0 + u * LPB + gl);
+        d[u] = h0 == 0 ? dsc[u] : __builtin_amdgcn_raw_buffer_load_b32(hr, (hoff + r) * 4u, 0, NT);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const bool valid = h0 + u * LPB + gl < nh;
+        const unsigned int l1 = d[u] & 3u;
+        const unsigned long long bv = __ballot(valid) & gmask;
+        const unsigned long long b0 = __ballot(valid && (l1 & 1u)) & gmask;
+        const unsigned long long b1 = __ballot(valid && (l1 & 2u)) & gmask;
+        off[u] = base + pm_lanes_below(bv) + pm_lanes_below(b0) + 2u * pm_lanes_below(b1);
+        len[u] = valid ? (int)l1 + 1 : 0;
+        base += (unsigned int)(__popcll(bv) + __popcll(b0) + 2 * __popcll(b1));
+      }
+      typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+      u4 wv[U][4][VPR];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const unsigned int o = (voff + off[u]) * (unsigned int)sizeof(VT);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+          for (int hh = 0; hh < VPR; hh++)
+            wv[u][q][hh] = __builtin_bit_cast(
+                u4, __builtin_amdgcn_raw_buffer_load_b128(vr, o + (unsigned int)q * rowb + 16u * hh,
+                                                          0, NT));
+      }
+      double xv[U][4];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const unsigned int col = d[u] >> 2;
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) {
+          const double xc = pm_bload<double, 0>(xr, (col + (unsigned int)kk) * 8u);
+          xv[u][kk] = kk < len[u] ? xc : 0.0;
+        }
+      }
+      if (h0 == 0)  // the next blocks' first descriptors, behind this pass's loads
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          dnext[u] = __builtin_amdgcn_raw_buffer_load_b32(
+              hrn, (hoffn + (unsigned int)(u * LPB + gl)) * 4u, 0, NT);
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          VT v[4];
+          __builtin_memcpy(v, &wv[u][q][0], sizeof(v));
+#pragma unroll
+          for (int kk = 0; kk < 4; kk++) a[q] += (double)v[kk] * xv[u][kk];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) dsc[u] = dnext[u];
+#pragma unroll
+    for (int q = 0; q < 4; q++) a[q] = pm_group_sum<LPB>(a[q]);
+    if (gl == LPB - 1) {
       for (int q = 0; q < n; q++) {
         double yq = a[q];
         if (SCALE_M) yq *= mq[q];
@@ -571,9 +616,9 @@ static int launch_spmv_shape(pm_ctx *c, int cfg, const VT *val, const double *x,
     PM_SPMV_CH_CASE(11, 1, true)
     PM_SPMV_CH_CASE(12, 2, true)
     case 15:
-      hipLaunchKernelGGL((k_spmv_ch2<VT, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream, A.nrn,
-                         A.neq, A.fr, A.nr, A.colptr, A.valptr, A.chptr, A.chunk, val, x, y, w, m,
-                         c->d_partials, S);
+      hipLaunchKernelGGL((k_spmv_chg<VT, 32, 1, NV, SCALE_M>), dim3(g), dim3(PM_WG), 0, c->stream,
+                         A.nrn, A.neq, A.fr, A.nr, A.colptr, A.valptr, A.chptr, A.chunk, val, x, y,
+                         w, m, c->d_partials, S);
       break;
     default: PM_SPMV_CASE(5, 32, 4)
   }

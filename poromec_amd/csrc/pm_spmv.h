@@ -17,8 +17,8 @@
 // PF 2: one run of <= 4 consecutive columns per lane, UNR runs in flight
 // (k_spmv_ch); PF 3: the same with the XCD's workgroups sweeping its blocks
 // together; PF 4: PF 1 sweeping; PF 5: runs, one or two per lane by block
-// width, sweeping (k_spmv_cha); PF 6: runs, one per lane, 32 lanes per block
-// (two blocks per wavefront), sweeping (k_spmv_ch2)
+// width, sweeping (k_spmv_cha); PF 6: runs, LANES lanes per block (two
+// blocks per wavefront), UNR runs per lane and pass, sweeping (k_spmv_chg)
 #define PM_SPMV_NCFG 16
 static constexpr int pm_spmv_lanes[PM_SPMV_NCFG] = {64, 64, 64, 64, 64, 32, 64, 64,
                                                     64, 64, 64, 64, 64, 64, 64, 32};
@@ -70,15 +70,15 @@ __device__ __forceinline__ double pm_dpp(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// sum over each group of LANES (32 or 64) lanes with DPP moves instead of
-// LDS permutes; the sum is valid in the group's last lane (31 / 63)
+// sum over each group of LANES (16, 32 or 64) lanes with DPP moves instead
+// of LDS permutes; the sum is valid in the group's last lane (15, 31 / 63)
 template <int LANES>
 __device__ __forceinline__ double pm_group_sum(double v) {
   v += pm_dpp<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
   v += pm_dpp<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
   v += pm_dpp<0x124, 0xf>(v);  // row_ror:4
   v += pm_dpp<0x128, 0xf>(v);  // row_ror:8  -> every lane holds its row of 16
-  v += pm_dpp<0x142, 0xa>(v);  // row_bcast:15 -> rows 1, 3 hold rows 0+1, 2+3
+  if (LANES >= 32) v += pm_dpp<0x142, 0xa>(v);  // row_bcast:15 -> rows 1, 3 hold rows 0+1, 2+3
   if (LANES == 64) v += pm_dpp<0x143, 0xc>(v);  // row_bcast:31 -> row 3 holds all
   return v;
 }
