@@ -212,7 +212,8 @@ def batch_form(args):
     from poromec_amd import problem as P
     from poromec_amd.solver import MefSolver, SolverConfig
 
-    decks = P.batch_decks(nbox=args.nbox, seed=args.seed, nmin=args.nmin, nmax=args.nmax)
+    nen = 8 if args.element == "q1" else 20
+    decks = P.batch_decks(nbox=args.nbox, seed=args.seed, nmin=args.nmin, nmax=args.nmax, nen=nen)
     big, off = P.concat_decks(decks)
     t0 = time.perf_counter()
     s = MefSolver(big)
@@ -222,7 +223,8 @@ def batch_form(args):
     setup_s = time.perf_counter() - t0
     s.pform(2, True, False, big.dt, big.dt)
     shapes = {name: s.tune_spmv(use32, reps=5)[0] for use32, name in ((False, "fp64"), (True, "fp32"))}
-    log(f"batch: {args.nbox} boxes, neq {s.neq}, setup {setup_s:.1f} s, shapes {shapes}")
+    log(f"batch ({args.element}): {args.nbox} boxes, neq {s.neq}, setup {setup_s:.1f} s, "
+        f"shapes {shapes}")
     out = {}
     forms = [f for f in (("fp64", SolverConfig(tol=1e-10)),
                          ("fp32ir", SolverConfig(tol=1e-10, precision="fp32ir")))
@@ -263,7 +265,7 @@ def batch_form(args):
         seq = {"boxes_timed": k, "box_steps_per_s": k * args.steps / wall,
                "s_per_step_all_boxes": wall / max(args.steps, 1)}
         log(f"  one box at a time (fp64, first {k} boxes): {seq}")
-    print(json.dumps({"form": "batch", "graphs": not args.batch_eager,
+    print(json.dumps({"form": "batch", "element": args.element, "graphs": not args.batch_eager,
                       "nbox": args.nbox, "n_range": [args.nmin, args.nmax],
                       "neq": s.neq, "steps": args.steps, "warmup": args.warmup,
                       "setup_s": setup_s, "spmv_shapes": shapes, **out,
@@ -281,8 +283,9 @@ def adjoint_form(args):
     from poromec_amd.solver import MefSolver, SolverConfig
 
     n = args.n if args.n is not None else 87
+    nen = 8 if args.element == "q1" else 20
     deck = P.heterogeneous_box_deck(n, seed=args.seed, dt=0.1, nsteps=args.nsteps,
-                                    corr_len=min(8.0, n / 2))
+                                    corr_len=min(8.0, n / 2), nen=nen)
     cfg = SolverConfig(tol=1e-13, maxit=200000)
     rng = np.random.default_rng(args.seed + 1)
     probe = MefSolver(deck)
@@ -290,7 +293,7 @@ def adjoint_form(args):
     free_p = np.nonzero(idm[: deck.nnodev, 3] > 0)[0]
     sens = [(int(v), 3) for v in rng.choice(free_p, size=min(args.nsens, len(free_p)), replace=False)]
     tdeck = P.heterogeneous_box_deck(n, seed=args.seed + 100, dt=0.1, nsteps=args.nsteps,
-                                     corr_len=min(8.0, n / 2))
+                                     corr_len=min(8.0, n / 2), nen=nen)
     probe.set_materials(tdeck)
     obs = np.zeros((args.nsteps, len(sens)))
     for k in range(args.nsteps):
@@ -303,6 +306,8 @@ def adjoint_form(args):
     t0 = time.perf_counter()
     J, gE, gk = m._evaluate(lE, lk)
     t_adj = time.perf_counter() - t0
+    log(f"adjoint {n}^3 ({args.element}), {args.nsteps} steps, checkpoints every {args.ckpt}: "
+        f"J {J:.12e}, forward plus adjoint {t_adj:.1f} s")
     g = np.concatenate([gE, gk])
     h, errs = args.fd_h, []
     for _ in range(args.ndir):
@@ -312,7 +317,8 @@ def adjoint_form(args):
         fd = (jp - jm) / (2 * h)
         errs.append(abs(fd - g @ v) / max(abs(fd), 1e-300))
         log(f"  direction {len(errs)}: fd {fd:.12e} adjoint {g @ v:.12e} rel {errs[-1]:.2e}")
-    print(json.dumps({"form": "adjoint", "n": n, "numel": deck.numel, "neq": m.solver.neq,
+    print(json.dumps({"form": "adjoint", "element": args.element, "n": n, "numel": deck.numel,
+                      "neq": m.solver.neq,
                       "nsteps": args.nsteps, "nsens": len(sens), "ckpt_every": args.ckpt,
                       "J": J, "forward_plus_adjoint_s": t_adj, "fd_h": h, "ndir": args.ndir,
                       "fd_rel_err_max": max(errs) if errs else None,
@@ -471,6 +477,10 @@ def main():
     ap.add_argument("--ckpt", type=int, default=10)
     ap.add_argument("--ndir", type=int, default=20)
     ap.add_argument("--fd-h", type=float, default=1e-6)
+    ap.add_argument("--no-q1", action="store_true",
+                    help="skip the Q1/Q1 128^3 leg the default run adds when time is left")
+    ap.add_argument("--q1-reserve", type=float, default=90.0,
+                    help="seconds of budget the Q1/Q1 128^3 leg needs to start")
     ap.add_argument("--spmv-only", action="store_true",
                     help="set up, then only time the FP64 / FP32 SpMV and a copy kernel "
                          "(the run profiled with hardware counters)")
@@ -721,9 +731,59 @@ def main():
                      "frac_compulsory": b32 / ms32 / 1e6 / HBM_PEAK_GBPS,
                      "achieved_traffic": (traffic32 / ms32 / 1e6) if traffic32 else None},
         "cpu_baseline": cpu,
-        "wall_s": time.perf_counter() - T_START,
     }
+    # the configuration's own 128^3 mesh at its nominal 8.6M dofs (the Q1/Q1
+    # element), one step of each form, when the budget still has room for it
+    if not q1 and args.n is None and not args.no_q1 and args.only is None:
+        if left() - margin >= args.q1_reserve:
+            solver.close()
+            out["q1q1_128cube"] = q1_nominal_leg(args, cfg32, cfg64, left, margin)
+        else:
+            log(f"Q1/Q1 128^3 leg skipped: {left():.0f} s left of the {budget:.0f} s budget")
+    out["wall_s"] = time.perf_counter() - T_START
     print(json.dumps(out), flush=True)
+
+
+def q1_nominal_leg(args, cfg32, cfg64, left, margin):
+    """One FP32+IR and one FP64 step, each from the zero state, of the
+    128^3-cell box with the equal-order Q1/Q1 element (8,586,756 dofs before
+    restraints): the benchmark configuration's mesh at its nominal size."""
+    from poromec_amd import problem as P
+    from poromec_amd.solver import MefSolver
+    t0 = time.perf_counter()
+    d = P.heterogeneous_box_deck(128, seed=args.seed, dt=0.1, patch_fraction=0.1, nsteps=50,
+                                 nen=8)
+    s = MefSolver(d)
+    info = s.matrix_info()
+    s.pform(2, True, False, d.dt, d.dt)
+    shapes = {name: s.tune_spmv(use32, reps=5)[0] for use32, name in ((False, "fp64"), (True, "fp32"))}
+    res = {"workload": f"Biot u-p Q1/Q1 hexa8 box 128^3 cells ({s.neq} equations), "
+                       "heterogeneous log-k, 10% patch load, dt 0.1 s, SQMR+Jacobi 1e-10",
+           "neq": s.neq, "nnz": info["nnz"], "setup_s": time.perf_counter() - t0,
+           "spmv_shapes": shapes}
+    for name, cfg in (("fp32ir", cfg32), ("fp64", cfg64)):
+        if name == "fp64" and left() - margin < 1.8 * res["fp32ir"]["s_per_step"]:
+            log(f"Q1/Q1 128^3 FP64 step skipped: {left():.0f} s left")
+            break
+        z4, zn = np.zeros((s.nnode, 4)), np.zeros(s.nnode)
+        s.set_state(u=z4, u0=z4, dp=zn, porosity=zn, t=0.0)
+        s.spmv_timing(args.live_every)
+        t1 = time.perf_counter()
+        inf = s.step(cfg)
+        wall = time.perf_counter() - t1
+        live = s.spmv_timing(0)
+        key = "fp32" if name == "fp32ir" else "fp64"
+        ms = live[key + "_ms"] / max(live[key + "_n"], 1)
+        comp = spmv_bytes(info, 4 if name == "fp32ir" else 8, fused_dot=True,
+                          shape=shapes[key])
+        res[name] = {"s_per_step": wall, "time_steps_per_s": 1.0 / wall,
+                     "krylov_iters": sum(inf["iters"]), "max_true_rel": max(inf["true_rel"]),
+                     "spmv_ms": ms, "spmv_GBps_compulsory": comp / ms / 1e6}
+        log(f"  Q1/Q1 128^3 {name}: {wall:.1f} s, iters {inf['iters']}, SpMV {ms:.3f} ms")
+    if "fp64" in res:
+        res["speedup_fp32ir_vs_fp64"] = res["fp64"]["s_per_step"] / res["fp32ir"]["s_per_step"]
+    s.close()
+    return res
 
 
 if __name__ == "__main__":
