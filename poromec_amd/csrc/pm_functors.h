@@ -177,3 +177,59 @@ struct SqmrQ {
   }
 };
 
+// The same iteration in two vector passes instead of three (the single-system
+// loop; the batched loop keeps SqmrR / SqmrD / SqmrQ for its per-box
+// epilogues).  The first pass does SqmrR's update and the part of SqmrD that
+// needs only the new r (t = r m and r.t), so one reduction carries both
+// dots; the second pass does the rest of SqmrD (d, x) and SqmrQ.  Every
+// value is formed by the same expression and every sum over the same rows in
+// the same order as in the three-pass form, so the results are the same to
+// the bit.  Convergence and breakdown are held in `pending` until the second
+// pass has made this iteration's update of x, as the three-pass form does.
+struct SqmrRT {
+  double *r, *t;
+  const double *m;
+  __device__ void operator()(int64_t i, const PmScalars &s, double *acc) const {
+    double ri = r[i] - s.s[S_ALPHA] * t[i];
+    r[i] = ri;
+    acc[0] += ri * ri;
+    double ti = ri * m[i];
+    t[i] = ti;
+    acc[1] += ri * ti;
+  }
+};
+struct SqmrRTEpi {
+  __device__ void operator()(PmScalars *S, const double *r) const {
+    SqmrREpi{}(S, r);
+    if (S->s[S_RO] == 0.0) {
+      S->pending = PM_ERR_BREAKDOWN;
+      return;
+    }
+    S->s[S_BETA] = r[1] / S->s[S_RO];
+    S->s[S_RO] = r[1];
+    S->resid = S->s[S_NORMR];
+    if (S->s[S_NORMR] < S->conv) S->pending = 1;
+  }
+};
+struct SqmrDXQ {
+  double *d, *x, *q;
+  const double *t;
+  __device__ void operator()(int64_t i, const PmScalars &s, double *) const {
+    double qi = q[i];
+    double di = s.s[S_T1] * d[i] + s.s[S_T2] * qi;
+    d[i] = di;
+    x[i] = x[i] + di;
+    q[i] = t[i] + s.s[S_BETA] * qi;
+  }
+};
+struct SqmrFinishEpi {  // a pending convergence or breakdown takes effect
+  __device__ void operator()(PmScalars *S, const double *) const {
+    if (S->pending == 1) {
+      S->done = 1;
+      S->iter = S->it;
+    } else if (S->pending < 0) {
+      S->done = S->pending;
+    }
+  }
+};
+

@@ -396,7 +396,10 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_chg(
   if (S != nullptr && S->done) return;
   constexpr int G = 64 / LPB;  // blocks per wavefront
   constexpr int GROUPS = PM_WG / 64;
-  constexpr int NT = 2;
+  // FP32 values stream with the non-temporal hint; FP64 values (two 16-byte
+  // loads per row and run, into the same cache lines) without it: at 87^3 the
+  // hint cost the FP64 product 13 % (3.71 against 3.23 ms) and left FP32 within 1 %
+  constexpr int NT = sizeof(VT) == 4 ? 2 : 0;
   constexpr int VPR = sizeof(VT) == 4 ? 1 : 2;  // 16-byte loads per row and run
   const __amdgpu_buffer_rsrc_t xr = pm_rsrc(x, (unsigned int)neq * 8u);
   const int lane = threadIdx.x % 64;
@@ -1269,9 +1272,8 @@ static int run_sqmr(pm_ctx *c, int n, const double *b, double *x, const double *
       c->cur_iter = launched + it + 1;
       launch_spmv<1, false>(c, use32, q, t, q, nullptr, S, SqmrSigma{});
       (use32 ? info->spmv32 : info->spmv64)++;
-      launch_vec<1>(c, n, S, SqmrR{r, t}, SqmrREpi{});
-      launch_vec<1>(c, n, S, SqmrD{d, x, t, q, r, m}, SqmrDEpi{});
-      launch_vec<0>(c, n, S, SqmrQ{q, t}, NoEpi{});
+      launch_vec<2>(c, n, S, SqmrRT{r, t, m}, SqmrRTEpi{});
+      launch_vec<0>(c, n, S, SqmrDXQ{d, x, q, t}, SqmrFinishEpi{});
     }
     launched += chunk;
     PM_CHECK_HIP(hipGetLastError());
