@@ -394,9 +394,8 @@ static void bchunk(pm_ctx *c, int solver, int use32, int chunk, const double *m,
     } else {
       double *t = w[0], *r = w[1], *q = w[2], *d = w[3];
       bspmv<1>(c, use32, q, t, q, CountDone<SqmrSigma>{SqmrSigma{}, B->d_ndone});
-      bvec<1>(c, SqmrR{r, t}, SqmrREpi{});
-      bvec<1>(c, SqmrD{d, x, t, q, r, m}, CountDone<SqmrDEpi>{SqmrDEpi{}, B->d_ndone});
-      bvec<0>(c, SqmrQ{q, t}, NoEpi{});
+      bvec<2>(c, SqmrRT{r, t, m}, SqmrRTEpi{});
+      bvec<0>(c, SqmrDXQ{d, x, q, t}, CountDone<SqmrFinishEpi>{SqmrFinishEpi{}, B->d_ndone});
     }
   }
 }

@@ -122,16 +122,15 @@ struct SqmrSigma {
     S->s[S_ALPHA] = S->s[S_RO] / sigma;
   }
 };
-struct SqmrR {
-  double *r;
-  const double *t;
-  __device__ void operator()(int64_t i, const PmScalars &s, double *acc) const {
-    double ri = r[i] - s.s[S_ALPHA] * t[i];
-    r[i] = ri;
-    acc[0] += ri * ri;
-  }
-};
-struct SqmrREpi {
+// After t = A q and alpha (SqmrSigma), an iteration is two vector passes.
+// SqmrRT: r -= alpha t, then t = M r, with r.r and r.t in one reduction
+// (SqmrRTEpi: tau, c and the coefficients of d, then rho, beta and the stop
+// test).  SqmrDXQ: d = T1 d + T2 q, x += d, q = t + beta q.  Each value is
+// the expression of the reference's loop (Iters.f:6100-6436) and each dot
+// runs over the rows in order, as the reference's separate loops do.
+// Convergence and breakdown are held in `pending` until SqmrDXQ has made the
+// iteration's update of x; SqmrFinishEpi then sets `done`.
+struct SqmrREpi {  // the part of SqmrRTEpi that needs only r.r
   __device__ void operator()(PmScalars *S, const double *r) const {
     double vn = sqrt(r[0]) / S->s[S_TAU];
     double cn = 1.0 / sqrt(1.0 + vn * vn);
@@ -142,50 +141,6 @@ struct SqmrREpi {
     S->s[S_NORMR] = sqrt(r[0]);
   }
 };
-struct SqmrD {
-  double *d, *x, *t;
-  const double *q, *r, *m;
-  __device__ void operator()(int64_t i, const PmScalars &s, double *acc) const {
-    double di = s.s[S_T1] * d[i] + s.s[S_T2] * q[i];
-    d[i] = di;
-    x[i] = x[i] + di;
-    double ri = r[i], ti = ri * m[i];
-    t[i] = ti;
-    acc[0] += ri * ti;
-  }
-};
-struct SqmrDEpi {
-  __device__ void operator()(PmScalars *S, const double *r) const {
-    if (S->s[S_RO] == 0.0) {
-      S->done = PM_ERR_BREAKDOWN;
-      return;
-    }
-    S->s[S_BETA] = r[0] / S->s[S_RO];
-    S->s[S_RO] = r[0];
-    S->resid = S->s[S_NORMR];
-    if (S->s[S_NORMR] < S->conv) {
-      S->done = 1;
-      S->iter = S->it;
-    }
-  }
-};
-struct SqmrQ {
-  double *q;
-  const double *t;
-  __device__ void operator()(int64_t i, const PmScalars &s, double *) const {
-    q[i] = t[i] + s.s[S_BETA] * q[i];
-  }
-};
-
-// The same iteration in two vector passes instead of three (the single-system
-// loop; the batched loop keeps SqmrR / SqmrD / SqmrQ for its per-box
-// epilogues).  The first pass does SqmrR's update and the part of SqmrD that
-// needs only the new r (t = r m and r.t), so one reduction carries both
-// dots; the second pass does the rest of SqmrD (d, x) and SqmrQ.  Every
-// value is formed by the same expression and every sum over the same rows in
-// the same order as in the three-pass form, so the results are the same to
-// the bit.  Convergence and breakdown are held in `pending` until the second
-// pass has made this iteration's update of x, as the three-pass form does.
 struct SqmrRT {
   double *r, *t;
   const double *m;
