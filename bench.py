@@ -646,7 +646,8 @@ def main():
     ms64 = live64["fp64_ms"] / max(live64["fp64_n"], 1) if live64 else None
     alg32 = spmv_alg_bytes(info, 8 if args.only == "fp64" else 4)
     alg64 = spmv_alg_bytes(info, 8)
-    log(f"SpMV in the timed steps (every {args.live_every}th launch timed): fp32 {ms32:.3f} ms x "
+    first = "fp64" if args.only == "fp64" else "fp32"
+    log(f"SpMV in the timed steps (every {args.live_every}th launch timed): {first} {ms32:.3f} ms x "
         f"{live32['fp32_n']}" + (f", fp64 {ms64:.3f} ms x {live64['fp64_n']}" if ms64 else ""))
 
     steps32, steps64 = len(walls32), len(walls64)
