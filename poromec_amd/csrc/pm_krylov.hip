@@ -293,12 +293,12 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_cha(
     const PmRunMeta nn = pm_run_meta(k + 2 * kstep, k1, colptr, valptr, nr, chptr);
     const int n = cur.n;
     const int f = fr[k];
+    // the rows' w and m at wave-uniform addresses (scalar loads)
     double wq[4] = {0.0, 0.0, 0.0, 0.0}, mq[4] = {1.0, 1.0, 1.0, 1.0};
-    if (lane == 63) {
-      for (int q = 0; q < n; q++) {
-        if (NV >= 1) wq[q] = w[f + q];
-        if (SCALE_M) mq[q] = m[f + q];
-      }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (NV >= 1 && q < n) wq[q] = w[f + q];
+      if (SCALE_M && q < n) mq[q] = m[f + q];
     }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     const __amdgpu_buffer_rsrc_t vr = pm_rsrc(
