@@ -324,16 +324,22 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_cha(
     (void)base;
     dsc[0] = dn[0];
     dsc[1] = dn[1];
+    // the four rows' sums together; lanes 0-3 hold rows 0, 2, 1, 3
+    const double sr = pm_rows4_sum<64>(lane, a);
+    const int qr = pm_row_of(lane);
+    if (lane < 4 && qr < n) {
+      double wl = wq[0], ml = mq[0];
 #pragma unroll
-    for (int q = 0; q < 4; q++) a[q] = pm_group_sum<64>(a[q]);
-    if (lane == 63) {
-      for (int q = 0; q < n; q++) {
-        double yq = a[q];
-        if (SCALE_M) yq *= mq[q];
-        y[f + q] = yq;
-        if (NV >= 1) d0 += yq * wq[q];
-        if (NV >= 2) d1 += yq * yq;
-      }
+      for (int q = 1; q < 4; q++)
+        if (qr == q) {
+          wl = wq[q];
+          ml = mq[q];
+        }
+      double yq = sr;
+      if (SCALE_M) yq *= ml;
+      y[f + qr] = yq;
+      if (NV >= 1) d0 += yq * wl;
+      if (NV >= 2) d1 += yq * yq;
     }
     cur = nx;
     nx = nn;
@@ -443,8 +449,17 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_chg(
     const PmRunMeta me = pm_pick<G>(cur, gi);
     const int n = me.n, nh = me.nh;
     const int f = n > 0 ? fr[k] : 0;
+    // w and m of the rows this lane writes: one row per lane 0-3 of the group
+    // (transposed reduction, 32 or 64 lanes), all rows in its last lane
+    // otherwise
     double wq[4] = {0.0, 0.0, 0.0, 0.0}, mq[4] = {1.0, 1.0, 1.0, 1.0};
-    if (gl == LPB - 1) {
+    if constexpr (LPB == 32 || LPB == 64) {
+      const int qr = pm_row_of(gl);
+      if (gl < 4 && qr < n) {
+        if (NV >= 1) wq[0] = w[f + qr];
+        if (SCALE_M) mq[0] = m[f + qr];
+      }
+    } else if (gl == LPB - 1) {
       for (int q = 0; q < n; q++) {
         if (NV >= 1) wq[q] = w[f + q];
         if (SCALE_M) mq[q] = m[f + q];
@@ -527,15 +542,28 @@ __global__ __launch_bounds__(PM_WG, sizeof(VT) == 4 ? 6 : 1) void k_spmv_chg(
     }
 #pragma unroll
     for (int u = 0; u < U; u++) dsc[u] = dnext[u];
-#pragma unroll
-    for (int q = 0; q < 4; q++) a[q] = pm_group_sum<LPB>(a[q]);
-    if (gl == LPB - 1) {
-      for (int q = 0; q < n; q++) {
-        double yq = a[q];
-        if (SCALE_M) yq *= mq[q];
-        y[f + q] = yq;
-        if (NV >= 1) d0 += yq * wq[q];
+    if constexpr (LPB == 32 || LPB == 64) {
+      // the four rows' sums together; the group's lanes 0-3 hold rows 0, 2, 1, 3
+      const double sr = pm_rows4_sum<LPB>(gl, a);
+      const int qr = pm_row_of(gl);
+      if (gl < 4 && qr < n) {
+        double yq = sr;
+        if (SCALE_M) yq *= mq[0];
+        y[f + qr] = yq;
+        if (NV >= 1) d0 += yq * wq[0];
         if (NV >= 2) d1 += yq * yq;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) a[q] = pm_group_sum<LPB>(a[q]);
+      if (gl == LPB - 1) {
+        for (int q = 0; q < n; q++) {
+          double yq = a[q];
+          if (SCALE_M) yq *= mq[q];
+          y[f + q] = yq;
+          if (NV >= 1) d0 += yq * wq[q];
+          if (NV >= 2) d1 += yq * yq;
+        }
       }
     }
     cur = nx;
